@@ -1,0 +1,742 @@
+// C ABI: device context, mesh upload, granular operators and the fused
+// iteration loop (Solver::solve, Solver/src/Solver.cpp:36-98).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/euler3d.h"
+#include "abi_internal.hpp"
+#include "device.hpp"
+#include "mesh_internal.hpp"
+#include "physics.hpp"
+
+namespace e3d {
+void launch_save_w0(const Range&, const DMesh&, const DState&, double, hipStream_t);
+void launch_spectral_radius(const Range&, const DMesh&, const DState&, double, hipStream_t);
+void launch_timestep(const Range&, const DMesh&, const DState&, double, hipStream_t);
+void launch_gradient(const Range&, const DMesh&, const DState&, int, hipStream_t);
+void launch_limiter(const Range&, const DMesh&, const DState&, int, double, double, hipStream_t);
+void launch_flux_conv(const Range&, const DMesh&, const DState&, int, const Gas&, hipStream_t);
+void launch_flux_diss(const Range&, const DMesh&, const DState&, int, const Gas&, double, hipStream_t);
+void launch_update(const Range&, const DMesh&, const DState&, double, double, hipStream_t);
+void launch_bc(const Range&, const DMesh&, const DState&, double, const double*, hipStream_t);
+void launch_exchange(int, const int*, const int*, int, double*, double*, double*, int, hipStream_t);
+void launch_rms_partials(const DMesh&, const DState&, hipStream_t);
+void launch_block_monitors(int, const int*, const int*, const int*, const int*, const int*, const int*,
+                           const double*, const DMesh&, const double*, const double*, double, double*, hipStream_t);
+void launch_interp_fused(const DMesh&, const DState&, int, int, bool, double, double, double, double, double,
+                         hipStream_t);
+void launch_prep_fused(const DMesh&, const DState&, double, double, double, hipStream_t);
+void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, const Gas&, double, double,
+                        const double*, hipStream_t);
+}  // namespace e3d
+
+using namespace e3d;
+
+namespace {
+enum KernelId { KI_INTERP, KI_PREP, KI_FLUX_UPDATE, KI_EXCHANGE, KI_MONITORS, KI_COUNT };
+const char* kKernelNames[KI_COUNT] = {"interp", "prep", "flux_update", "exchange", "monitors"};
+}  // namespace
+
+struct e3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool have_params = false, have_mesh = false;
+    e3d_params params{};
+    double fs[5] = {0, 0, 0, 0, 0};
+    double alpha[5] = {0, 0, 0, 0, 0}, beta[5] = {0, 0, 0, 0, 0};
+    Gas gas{};
+    // layout
+    int nb = 0;
+    std::vector<int> nreal, nall, roff, goff, foff, ghost_count;
+    DMesh m{};
+    DState s{};
+    // device allocations
+    std::vector<void*> allocs;
+    int* d_itf_dst = nullptr;
+    int* d_itf_org = nullptr;
+    int n_itf = 0;
+    int *d_tile0 = nullptr, *d_ntile = nullptr, *d_nreal = nullptr, *d_wall_off = nullptr, *d_wall_ghost = nullptr,
+        *d_wall_face = nullptr;
+    double* d_fcenter = nullptr;
+    double* d_mon = nullptr;   // [nb][16]
+    double* h_mon = nullptr;   // pinned
+    // monitors
+    int iter = 0;
+    bool started = false;      // initial exchange done
+    double rms0[5] = {0, 0, 0, 0, 0};
+    std::atomic<int> stop_request{0};
+    // timing
+    bool timing = false;
+    struct Ev {
+        hipEvent_t a, b;
+        int id;
+    };
+    std::vector<Ev> ev_pool;
+    size_t ev_used = 0;
+    double t_ms[KI_COUNT] = {0, 0, 0, 0, 0};
+    int t_n[KI_COUNT] = {0, 0, 0, 0, 0};
+};
+
+namespace e3d {
+void ctx_record_error(e3d_ctx* ctx, const std::string& msg) { ctx->err = msg; }
+const char* ctx_error_text(const e3d_ctx* ctx) { return ctx->err.empty() ? nullptr : ctx->err.c_str(); }
+}  // namespace e3d
+
+#define HIP_TRY(ctx, call)                                                                              \
+    do {                                                                                                \
+        hipError_t _e = (call);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            return set_error(ctx, E3D_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));         \
+    } while (0)
+
+namespace {
+
+void free_device(e3d_ctx* c) {
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    if (c->h_mon) (void)hipHostFree(c->h_mon);
+    c->h_mon = nullptr;
+    c->have_mesh = false;
+}
+
+template <class T>
+int dalloc(e3d_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return set_error(c, E3D_ENOMEM, "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    e = hipMemsetAsync(q, 0, bytes, c->stream);
+    if (e != hipSuccess) return set_error(c, E3D_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return E3D_OK;
+}
+
+template <class T>
+int upload(e3d_ctx* c, T** p, const std::vector<T>& h) {
+    int rc = dalloc(c, p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) HIP_TRY(c, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return E3D_OK;
+}
+
+int global_cell(const e3d_ctx* c, int b, int local) {
+    return local < c->nreal[b] ? c->roff[b] + local : c->m.NRp + c->goff[b] + (local - c->nreal[b]);
+}
+
+Range block_range(const e3d_ctx* c, int b) {
+    if (b < 0) return Range{0, c->m.NRp, 0, c->m.NG};
+    return Range{c->roff[b], c->roff[b] + c->nreal[b], c->goff[b], c->goff[b] + (c->nall[b] - c->nreal[b])};
+}
+
+int check_ready(e3d_ctx* c, int block) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    if (!c->have_mesh) return set_error(c, E3D_ESTATE, "no mesh uploaded");
+    if (!c->have_params) return set_error(c, E3D_ESTATE, "no parameters set");
+    if (block < -1 || block >= c->nb) return set_error(c, E3D_EINVAL, "block index out of range");
+    return E3D_OK;
+}
+
+int launch_status(e3d_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(c, E3D_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return E3D_OK;
+}
+
+void ev_begin(e3d_ctx* c, int id, size_t* slot) {
+    if (!c->timing) return;
+    if (c->ev_used == c->ev_pool.size()) {
+        e3d_ctx::Ev e;
+        (void)hipEventCreate(&e.a);
+        (void)hipEventCreate(&e.b);
+        c->ev_pool.push_back(e);
+    }
+    *slot = c->ev_used++;
+    c->ev_pool[*slot].id = id;
+    (void)hipEventRecord(c->ev_pool[*slot].a, c->stream);
+}
+void ev_end(e3d_ctx* c, size_t slot) {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev_pool[slot].b, c->stream);
+}
+// After a synchronisation: fold the recorded launches into the per-kernel totals.
+void ev_collect(e3d_ctx* c) {
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[i].a, c->ev_pool[i].b) == hipSuccess) {
+            c->t_ms[c->ev_pool[i].id] += ms;
+            c->t_n[c->ev_pool[i].id] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+// Residual norms and force coefficients of every block, summed over blocks in
+// block order (BlockCommunicator::getGlobal serial branch, :341-369), then the
+// drop and the stop test (PostProcessing::process, :220-308).
+int finish_monitor(e3d_ctx* c, e3d_monitor* out) {
+    size_t ev = 0;
+    ev_begin(c, KI_MONITORS, &ev);
+    const double dyn_head = 0.5 * c->params.gamma * c->params.mach * c->params.mach;
+    launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
+                          c->d_fcenter, c->m, c->s.W, c->s.part, dyn_head * c->params.cmac, c->d_mon, c->stream);
+    ev_end(c, ev);
+    int rc = launch_status(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->h_mon, c->d_mon, sizeof(double) * 16 * c->nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    const double aoa_rad = c->params.aoa_deg * 4. * std::atan(1.) / 180;
+    double sum[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < c->nb; ++b) {
+        const double* r = c->h_mon + 16 * b;
+        for (int v = 0; v < 5; ++v) sum[v] += r[v];
+        const double cl = r[5], cd = r[6];
+        sum[5] += cl * std::cos(aoa_rad) - cd * std::sin(aoa_rad);
+        sum[6] += cl * std::sin(aoa_rad) + cd * std::cos(aoa_rad);
+        sum[7] += r[7];
+        sum[8] += r[8];
+        sum[9] += r[9];
+    }
+    if (c->iter == 0)
+        for (int v = 0; v < 5; ++v) c->rms0[v] = sum[v];
+    e3d_monitor m{};
+    m.iter = c->iter;
+    for (int v = 0; v < 5; ++v) {
+        m.rms[v] = sum[v];
+        m.drop[v] = std::log10(sum[v]) - std::log10(c->rms0[v]);
+    }
+    m.cl = sum[5];
+    m.cd = sum[6];
+    m.cmx = sum[7];
+    m.cmy = sum[8];
+    m.cmz = sum[9];
+    m.stop = (c->iter + 1 == c->params.max_iter) || (m.drop[0] <= c->params.criterion) ? 1 : 0;
+    if (out) *out = m;
+    ++c->iter;
+    for (int v = 0; v < 5; ++v)
+        if (!std::isfinite(sum[v])) return set_error(c, E3D_ENONFINITE, "residual norm is not finite at iteration " +
+                                                                             std::to_string(m.iter));
+    return E3D_OK;
+}
+
+int exchange(e3d_ctx* c, int copy_all) {
+    size_t ev = 0;
+    ev_begin(c, KI_EXCHANGE, &ev);
+    launch_exchange(c->n_itf, c->d_itf_dst, c->d_itf_org, c->m.NA, c->s.W, c->s.grad, c->s.lim, copy_all, c->stream);
+    ev_end(c, ev);
+    return launch_status(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int e3d_create(int device, e3d_ctx** out) {
+    if (!out) return set_error(nullptr, E3D_EINVAL, "e3d_create: null output");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return set_error(nullptr, E3D_EHIP, std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "count 0"));
+    if (device < 0 || device >= n) return set_error(nullptr, E3D_EINVAL, "device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return set_error(nullptr, E3D_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    auto* c = new e3d_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return set_error(nullptr, E3D_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return E3D_OK;
+}
+
+void e3d_destroy(e3d_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_device(c);
+    for (auto& e : c->ev_pool) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int e3d_set_params(e3d_ctx* c, const e3d_params* p) {
+    if (!c || !p) return set_error(c, E3D_EINVAL, "e3d_set_params: null argument");
+    if (p->stages < 1 || p->stages > 5) return set_error(c, E3D_EINVAL, "RK stages must be 1..5");
+    if (p->order != 1 && p->order != 2) return set_error(c, E3D_EINVAL, "order must be 1 or 2");
+    if (p->flux != E3D_FLUX_ROE && p->flux != E3D_FLUX_AUSM) return set_error(c, E3D_EINVAL, "unknown flux scheme");
+    if (p->gradient != E3D_GRAD_GREEN_GAUSS && p->gradient != E3D_GRAD_LEAST_SQUARES)
+        return set_error(c, E3D_EINVAL, "unknown gradient");
+    if (p->limiter != E3D_LIM_BARTH_JESPERSEN && p->limiter != E3D_LIM_VENKATAKRISHNAN)
+        return set_error(c, E3D_EINVAL, "unknown limiter");
+    if (p->smoothing != 0) return set_error(c, E3D_EINVAL, "residual smoothing is not supported (NoSmoother only)");
+    if (!(p->gamma > 1.0)) return set_error(c, E3D_EINVAL, "gamma must exceed 1");
+    c->params = *p;
+    // RungeKutta constructor tables (Solver/src/RungeKutta.cpp:23-68)
+    static const double A[5][5] = {{1.0}, {0.6, 1.0}, {0.6, 0.6, 1.0}, {0.25, 1.0 / 3.0, 0.5, 1.0},
+                                   {0.2742, 0.2067, 0.5020, 0.5142, 1.0}};
+    static const double B[5][5] = {{1.0}, {1.0, 0.0}, {1.0, 0.0, 0.0}, {1.0, 0.5, 0.0, 0.0},
+                                   {1.0, 0.0, 0.56, 0.0, 0.44}};
+    for (int i = 0; i < 5; ++i) {
+        c->alpha[i] = A[p->stages - 1][i];
+        c->beta[i] = B[p->stages - 1][i];
+    }
+    // PrimitiveVariables::calculateFreeVariables (DataStructure/src/PrimitiveVariables.cpp:41-56)
+    const double pi = std::atan(1.0) * 4.0;
+    const double alpha_rad = p->aoa_deg * pi / 180.0;
+    const double cs = std::cos(alpha_rad), sn = std::sin(alpha_rad);
+    c->fs[0] = 1.0;
+    c->fs[1] = p->mach * std::sqrt(p->gamma) * cs;
+    c->fs[2] = p->mach * std::sqrt(p->gamma) * sn;
+    c->fs[3] = 0.0;
+    c->fs[4] = 1.0;
+    c->gas.gamma = p->gamma;
+    c->gas.g_over_gm1 = p->gamma / (p->gamma - 1.0);
+    c->gas.gm1 = p->gamma - 1.0;
+    c->have_params = true;
+    return E3D_OK;
+}
+
+int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
+    if (!c || !mesh) return set_error(c, E3D_EINVAL, "e3d_upload_mesh: null argument");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_device(c);
+    const int nb = (int)mesh->built.size();
+    c->nb = nb;
+    c->nreal.assign(nb, 0);
+    c->nall.assign(nb, 0);
+    c->roff.assign(nb, 0);
+    c->goff.assign(nb, 0);
+    c->foff.assign(nb, 0);
+    c->ghost_count.assign(nb, 0);
+    long NRp = 0, NG = 0, NF = 0;
+    int nslot = 1;
+    for (int b = 0; b < nb; ++b) {
+        const HostBlock& B = mesh->built[b];
+        c->nreal[b] = B.nreal;
+        c->nall[b] = B.nall;
+        c->roff[b] = (int)NRp;
+        c->goff[b] = (int)NG;
+        c->foff[b] = (int)NF;
+        c->ghost_count[b] = mesh->raw.topo[b].ghost_count;
+        NRp += (long)(B.nreal + kWG - 1) / kWG * kWG;
+        NG += B.nall - B.nreal;
+        NF += B.nface;
+        for (int q = 0; q < B.nreal; ++q) nslot = std::max(nslot, B.cell_off[q + 1] - B.cell_off[q]);
+    }
+    if (nslot > kMaxSlots) return set_error(c, E3D_EINVAL, "cells with more than 6 faces are not supported");
+    if (NRp + NG >= (1L << 31) - 1 || NF >= (1L << 31) - 1)
+        return set_error(c, E3D_EINVAL, "mesh too large for 32-bit cell indices");
+    DMesh& m = c->m;
+    m.NRp = (int)NRp;
+    m.NG = (int)NG;
+    m.NA = (int)(NRp + NG);
+    m.NF = (int)NF;
+    m.nslot = nslot;
+    const size_t NR = NRp, NA = m.NA;
+
+    std::vector<int> nf(NR, 0), nbr((size_t)nslot * NR, 0), fid((size_t)nslot * NR, 0);
+    std::vector<int> g_in(NG), g_face(NG), g_kind(NG);
+    std::vector<double> S(3 * (size_t)NF), area(NF), rL(3 * (size_t)NF), rR(3 * (size_t)NF), fc(3 * (size_t)NF);
+    std::vector<double> vol(NA, 1.0), lsqw((size_t)nslot * 3 * NR, 0.0);
+    std::vector<int> wg_block(NR / kWG, 0), tile0(nb), ntile(nb), wall_off(nb + 1, 0), wall_ghost, wall_face;
+    std::vector<int> itf_dst, itf_org;
+    for (int b = 0; b < nb; ++b) {
+        const HostBlock& B = mesh->built[b];
+        const auto& F = mesh->flat[b];
+        tile0[b] = c->roff[b] / kWG;
+        ntile[b] = (B.nreal + kWG - 1) / kWG;
+        for (int t = 0; t < ntile[b]; ++t) wg_block[tile0[b] + t] = b;
+        for (int f = 0; f < B.nface; ++f) {
+            const size_t g = c->foff[b] + (size_t)f;
+            for (int d = 0; d < 3; ++d) {
+                S[d * (size_t)NF + g] = B.S[3 * f + d];
+                rL[d * (size_t)NF + g] = B.rL[3 * f + d];
+                rR[d * (size_t)NF + g] = B.rR[3 * f + d];
+                fc[d * (size_t)NF + g] = B.fcenter[3 * f + d];
+            }
+            area[g] = B.area[f];
+        }
+        for (int q = 0; q < B.nreal; ++q) {
+            const int gc = c->roff[b] + q;
+            nf[gc] = B.cell_off[q + 1] - B.cell_off[q];
+            vol[gc] = B.volume[q];
+            for (int k = 0; k < nf[gc]; ++k) {
+                const int f = B.cell2face[B.cell_off[q] + k];
+                const bool right = B.face2cell[2 * f] != q;
+                nbr[(size_t)k * NR + gc] = global_cell(c, b, B.cell2cell[B.cell_off[q] + k]);
+                fid[(size_t)k * NR + gc] = (c->foff[b] + f) | (right ? (int)0x80000000u : 0);
+                for (int d = 0; d < 3; ++d)
+                    lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
+            }
+        }
+        for (size_t j = 0; j < B.bnds.size(); ++j) {
+            const bool real_bnd = (int)j < B.n_real_boundaries;
+            const BcType t = B.bnds[j].type;
+            for (int gl : B.bnds[j].ghosts) {
+                const int gi = c->goff[b] + (gl - B.nreal);
+                g_in[gi] = global_cell(c, b, B.cell2cell[B.cell_off[gl]]);
+                g_face[gi] = c->foff[b] + B.cell2face[B.cell_off[gl]];
+                int kind = 0;
+                if (real_bnd) {
+                    kind |= GK_REFLECT;
+                    if (t == BC_WALL || t == BC_SYMMETRY) kind |= GK_SLIP;
+                    else if (t == BC_FARFIELD) kind |= GK_FARFIELD;
+                }
+                if (t == BC_WALL) {
+                    kind |= GK_WALL;
+                    wall_ghost.push_back(m.NRp + gi);
+                    wall_face.push_back(g_face[gi]);
+                }
+                g_kind[gi] = kind;
+                vol[m.NRp + gi] = B.volume[gl];
+            }
+        }
+        wall_off[b + 1] = (int)wall_ghost.size();
+        // interfaces: destination ghosts numbered from GhostCount in topology order
+        int k = 0;
+        for (size_t i = 0; i < F.itf_origin_block.size(); ++i) {
+            const int o = F.itf_origin_block[i];
+            for (int q = F.itf_off[i]; q < F.itf_off[i + 1]; ++q, ++k) {
+                itf_dst.push_back(global_cell(c, b, c->ghost_count[b] + k));
+                itf_org.push_back(c->roff[o] + F.itf_origin_cells[q]);
+            }
+        }
+    }
+    c->n_itf = (int)itf_dst.size();
+    int rc = 0;
+    const int* cnf;
+    (void)cnf;
+    int *d_nf, *d_nbr, *d_fid, *d_gin, *d_gface, *d_gkind, *d_wgb;
+    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw;
+    if ((rc = upload(c, &d_nf, nf)) || (rc = upload(c, &d_nbr, nbr)) || (rc = upload(c, &d_fid, fid)) ||
+        (rc = upload(c, &d_gin, g_in)) || (rc = upload(c, &d_gface, g_face)) || (rc = upload(c, &d_gkind, g_kind)) ||
+        (rc = upload(c, &d_wgb, wg_block)) || (rc = upload(c, &d_S, S)) || (rc = upload(c, &d_area, area)) ||
+        (rc = upload(c, &d_rL, rL)) || (rc = upload(c, &d_rR, rR)) || (rc = upload(c, &d_vol, vol)) ||
+        (rc = upload(c, &d_lsqw, lsqw)) || (rc = upload(c, &c->d_fcenter, fc)) || (rc = upload(c, &c->d_itf_dst, itf_dst)) ||
+        (rc = upload(c, &c->d_itf_org, itf_org)) || (rc = upload(c, &c->d_tile0, tile0)) ||
+        (rc = upload(c, &c->d_ntile, ntile)) || (rc = upload(c, &c->d_nreal, c->nreal)) ||
+        (rc = upload(c, &c->d_wall_off, wall_off)) || (rc = upload(c, &c->d_wall_ghost, wall_ghost)) ||
+        (rc = upload(c, &c->d_wall_face, wall_face))) {
+        free_device(c);
+        return rc;
+    }
+    m.nf = d_nf; m.nbr = d_nbr; m.fid = d_fid; m.S = d_S; m.area = d_area; m.rL = d_rL; m.rR = d_rR;
+    m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
+    DState& s = c->s;
+    if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
+        (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
+        (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.lg, 15 * NA)) || (rc = dalloc(c, &s.dt, NR)) ||
+        (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / kWG + 1))) ||
+        (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb))) {
+        free_device(c);
+        return rc;
+    }
+    hipError_t e = hipHostMalloc((void**)&c->h_mon, sizeof(double) * 16 * nb, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_device(c);
+        return set_error(c, E3D_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_error(c, E3D_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    c->have_mesh = true;
+    c->iter = 0;
+    c->started = false;
+    return E3D_OK;
+}
+
+int e3d_nblocks(const e3d_ctx* c) { return c ? c->nb : E3D_EINVAL; }
+
+int e3d_init_freestream(e3d_ctx* c) {
+    if (!c || !c->have_mesh || !c->have_params) return set_error(c, E3D_ESTATE, "e3d_init_freestream: need mesh and params");
+    std::vector<double> w(c->m.NA);
+    for (int v = 0; v < 5; ++v) {
+        std::fill(w.begin(), w.end(), c->fs[v]);
+        HIP_TRY(c, hipMemcpyAsync(c->s.W + (size_t)v * c->m.NA, w.data(), sizeof(double) * c->m.NA,
+                                  hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->s.Wn + (size_t)v * c->m.NA, w.data(), sizeof(double) * c->m.NA,
+                                  hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->iter = 0;
+    c->started = false;
+    return E3D_OK;
+}
+
+namespace {
+// (device array, components, real-only?) of an E3D_ARR_* selector
+int array_desc(e3d_ctx* c, int which, double** base, int* ncomp, bool* real_only) {
+    switch (which) {
+        case E3D_ARR_W: *base = c->s.W; *ncomp = 5; *real_only = false; break;
+        case E3D_ARR_W0: *base = c->s.W0; *ncomp = 5; *real_only = true; break;
+        case E3D_ARR_CONV: *base = c->s.conv; *ncomp = 5; *real_only = true; break;
+        case E3D_ARR_DISS: *base = c->s.diss; *ncomp = 5; *real_only = true; break;
+        case E3D_ARR_GRAD: *base = c->s.grad; *ncomp = 15; *real_only = false; break;
+        case E3D_ARR_LIM: *base = c->s.lim; *ncomp = 5; *real_only = false; break;
+        case E3D_ARR_DT: *base = c->s.dt; *ncomp = 1; *real_only = true; break;
+        case E3D_ARR_SPECRAD: *base = c->s.specrad; *ncomp = 1; *real_only = true; break;
+        default: return set_error(c, E3D_EINVAL, "unknown array selector");
+    }
+    return E3D_OK;
+}
+
+int copy_array(e3d_ctx* c, int b, int which, double* host, bool to_device) {
+    if (!c || !c->have_mesh) return set_error(c, E3D_ESTATE, "no mesh uploaded");
+    if (b < 0 || b >= c->nb || !host) return set_error(c, E3D_EINVAL, "bad block or null buffer");
+    double* base;
+    int nc;
+    bool real_only;
+    int rc = array_desc(c, which, &base, &nc, &real_only);
+    if (rc) return rc;
+    const size_t stride = real_only ? (size_t)c->m.NRp : (size_t)c->m.NA;
+    const int nr = c->nreal[b], ng = c->nall[b] - c->nreal[b];
+    const size_t n_host = real_only ? nr : (size_t)c->nall[b];
+    const hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    for (int q = 0; q < nc; ++q) {
+        double* d_real = base + q * stride + c->roff[b];
+        double* h = host + q * n_host;
+        if (to_device) HIP_TRY(c, hipMemcpyAsync(d_real, h, sizeof(double) * nr, kind, c->stream));
+        else HIP_TRY(c, hipMemcpyAsync(h, d_real, sizeof(double) * nr, kind, c->stream));
+        if (!real_only && ng > 0) {
+            double* d_g = base + q * stride + c->m.NRp + c->goff[b];
+            if (to_device) HIP_TRY(c, hipMemcpyAsync(d_g, h + nr, sizeof(double) * ng, kind, c->stream));
+            else HIP_TRY(c, hipMemcpyAsync(h + nr, d_g, sizeof(double) * ng, kind, c->stream));
+        }
+    }
+    if (to_device && which == E3D_ARR_W) {  // keep the fused path's second buffer in step
+        for (int q = 0; q < 5; ++q) {
+            HIP_TRY(c, hipMemcpyAsync(c->s.Wn + q * stride + c->roff[b], c->s.W + q * stride + c->roff[b],
+                                      sizeof(double) * nr, hipMemcpyDeviceToDevice, c->stream));
+            if (ng > 0)
+                HIP_TRY(c, hipMemcpyAsync(c->s.Wn + q * stride + c->m.NRp + c->goff[b],
+                                          c->s.W + q * stride + c->m.NRp + c->goff[b], sizeof(double) * ng,
+                                          hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return E3D_OK;
+}
+}  // namespace
+
+int e3d_get_array(e3d_ctx* c, int block, int which, double* out) { return copy_array(c, block, which, out, false); }
+int e3d_set_array(e3d_ctx* c, int block, int which, const double* in) {
+    return copy_array(c, block, which, const_cast<double*>(in), true);
+}
+
+// ------------------------------------------------------------ granular ops
+
+int e3d_save_w0(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_save_w0(block_range(c, block), c->m, c->s, 1. / (c->params.gamma - 1.0), c->stream);
+    return launch_status(c);
+}
+
+int e3d_spectral_radius(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_spectral_radius(block_range(c, block), c->m, c->s, c->params.gamma, c->stream);
+    return launch_status(c);
+}
+
+int e3d_timestep(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_timestep(block_range(c, block), c->m, c->s, c->params.cfl, c->stream);
+    return launch_status(c);
+}
+
+int e3d_gradients(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_gradient(block_range(c, block), c->m, c->s, c->params.gradient, c->stream);
+    return launch_status(c);
+}
+
+int e3d_limiters(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    const double k = c->params.k;
+    launch_limiter(block_range(c, block), c->m, c->s, c->params.limiter == E3D_LIM_VENKATAKRISHNAN, k * k * k,
+                   c->params.omega, c->stream);
+    return launch_status(c);
+}
+
+int e3d_interpolation(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc || c->params.order != 2) return rc;
+    if ((rc = e3d_gradients(c, block))) return rc;
+    return e3d_limiters(c, block);
+}
+
+int e3d_flux_conv(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_flux_conv(block_range(c, block), c->m, c->s, c->params.flux, c->gas, c->stream);
+    return launch_status(c);
+}
+
+int e3d_flux_diss(e3d_ctx* c, int block, double beta) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_flux_diss(block_range(c, block), c->m, c->s, c->params.flux, c->gas, beta, c->stream);
+    return launch_status(c);
+}
+
+int e3d_smooth_residual(e3d_ctx* c, int block) { return check_ready(c, block); }
+
+int e3d_residual(e3d_ctx* c, int block, int stage) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    if (stage < 0 || stage >= c->params.stages) return set_error(c, E3D_EINVAL, "stage out of range");
+    if ((rc = e3d_interpolation(c, block)) || (rc = e3d_flux_conv(c, block)) ||
+        (rc = e3d_flux_diss(c, block, c->beta[stage])))
+        return rc;
+    return e3d_smooth_residual(c, block);
+}
+
+int e3d_update(e3d_ctx* c, int block, int stage) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    if (stage < 0 || stage >= c->params.stages) return set_error(c, E3D_EINVAL, "stage out of range");
+    launch_update(block_range(c, block), c->m, c->s, c->alpha[stage], c->gas.gm1, c->stream);
+    return launch_status(c);
+}
+
+int e3d_bc(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    launch_bc(block_range(c, block), c->m, c->s, c->params.gamma, c->fs, c->stream);
+    return launch_status(c);
+}
+
+int e3d_runge_kutta(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    for (int st = 0; st < c->params.stages; ++st)
+        if ((rc = e3d_residual(c, block, st)) || (rc = e3d_update(c, block, st)) || (rc = e3d_bc(c, block))) return rc;
+    return E3D_OK;
+}
+
+int e3d_exchange(e3d_ctx* c) {
+    int rc = check_ready(c, -1);
+    if (rc) return rc;
+    c->started = true;
+    return exchange(c, 1);
+}
+
+int e3d_monitors(e3d_ctx* c, e3d_monitor* out) {
+    int rc = check_ready(c, -1);
+    if (rc) return rc;
+    launch_rms_partials(c->m, c->s, c->stream);
+    if ((rc = launch_status(c))) return rc;
+    return finish_monitor(c, out);
+}
+
+int e3d_reset_monitors(e3d_ctx* c) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    c->iter = 0;
+    c->started = false;
+    for (double& r : c->rms0) r = 0.0;
+    return E3D_OK;
+}
+
+int e3d_request_stop(e3d_ctx* c) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    c->stop_request.store(1);
+    return E3D_OK;
+}
+
+// ------------------------------------------------------------- fused path
+
+int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
+    int rc = check_ready(c, -1);
+    if (rc) return rc;
+    if (n < 0) return set_error(c, E3D_EINVAL, "negative iteration count");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const e3d_params& p = c->params;
+    const bool order2 = p.order == 2;
+    const double k = p.k;
+    if (!c->started) {  // Solver.cpp:48
+        if ((rc = exchange(c, 0))) return rc;
+        c->started = true;
+    }
+    int done = 0;
+    for (int it = 0; it < n; ++it) {
+        if (c->stop_request.exchange(0)) break;
+        for (int st = 0; st < p.stages; ++st) {
+            size_t ev = 0;
+            if (order2) {
+                ev_begin(c, KI_INTERP, &ev);
+                launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, st == 0, k * k * k,
+                                    p.omega, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
+                ev_end(c, ev);
+            } else if (st == 0) {
+                ev_begin(c, KI_PREP, &ev);
+                launch_prep_fused(c->m, c->s, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
+                ev_end(c, ev);
+            }
+            ev_begin(c, KI_FLUX_UPDATE, &ev);
+            launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, c->gas, c->beta[st],
+                               c->alpha[st], c->fs, c->stream);
+            ev_end(c, ev);
+            if ((rc = launch_status(c))) return rc;
+            std::swap(c->s.W, c->s.Wn);
+        }
+        if ((rc = exchange(c, 0))) return rc;  // Solver.cpp:94
+        e3d_monitor mon;
+        rc = finish_monitor(c, &mon);
+        if (history) history[done] = mon;
+        ++done;
+        if (rc) return rc;
+        if (mon.stop) break;
+    }
+    return done;
+}
+
+// ------------------------------------------------------------- measurement
+
+int e3d_enable_timing(e3d_ctx* c, int on) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    c->timing = on != 0;
+    for (int i = 0; i < KI_COUNT; ++i) {
+        c->t_ms[i] = 0.0;
+        c->t_n[i] = 0;
+    }
+    c->ev_used = 0;
+    return E3D_OK;
+}
+
+int e3d_kernel_time(e3d_ctx* c, const char* name, double* avg_ms, int* launches) {
+    if (!c || !name || !avg_ms || !launches) return set_error(c, E3D_EINVAL, "e3d_kernel_time: null argument");
+    for (int i = 0; i < KI_COUNT; ++i)
+        if (std::strcmp(name, kKernelNames[i]) == 0) {
+            *launches = c->t_n[i];
+            *avg_ms = c->t_n[i] ? c->t_ms[i] / c->t_n[i] : 0.0;
+            return E3D_OK;
+        }
+    return set_error(c, E3D_EINVAL, std::string("unknown kernel name ") + name);
+}
+
+int e3d_sync(e3d_ctx* c) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    return E3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Device-side mesh description shared by the kernels and the host context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace e3d {
+
+constexpr int kWG = 256;          // threads per workgroup; block real ranges are padded to this
+constexpr int kMaxSlots = 6;      // faces per cell: tet 4, pyramid 5, hexahedron 6
+
+// Ghost kind bits.
+constexpr int GK_BC_MASK = 3;     // 0 none, 1 slip (wall/symmetry), 2 farfield
+constexpr int GK_SLIP = 1;
+constexpr int GK_FARFIELD = 2;
+constexpr int GK_REFLECT = 4;     // gradient reflected from the interior cell
+constexpr int GK_WALL = 8;        // marker is WALL (aerodynamic coefficients)
+
+// Global cell space: real cells of every block in [0, NRp) (block b from roff[b],
+// padded to kWG), then ghosts of every block in [NRp, NA). Arrays of real-cell
+// quantities have NRp entries, of all-cell quantities NA, of faces NF.
+// Per-cell face slots are slot-major: x[k*NRp + c], k < nslot, in the
+// reference's ascending face order.
+struct DMesh {
+    int NRp, NG, NA, NF, nslot;
+    const int* nf;        // [NRp] faces of the cell (0 = padding)
+    const int* nbr;       // [nslot][NRp] neighbour (global cell)
+    const int* fid;       // [nslot][NRp] global face, bit 31 set when the cell is the face's right cell
+    const double* S;      // [3][NF] area vectors
+    const double* area;   // [NF]
+    const double* rL;     // [3][NF]
+    const double* rR;     // [3][NF]
+    const double* vol;    // [NA]
+    const double* lsqw;   // [nslot*3][NRp] least-squares weights, (k*3+d)*NRp + c
+    const int* g_in;      // [NG] interior (global) of each ghost
+    const int* g_face;    // [NG] face of each ghost
+    const int* g_kind;    // [NG] GK_* bits
+    const int* wg_block;  // [NRp/kWG] block of each workgroup-sized tile
+};
+
+// Cell range of a granular launch: real cells [c0, c1) (global), ghosts
+// [g0, g1) (ghost index, global cell NRp + i). One block or all blocks.
+struct Range {
+    int c0, c1, g0, g1;
+};
+
+struct DState {
+    double* W;            // [5][NA] primitive state (current)
+    double* Wn;           // [5][NA] next-stage buffer of the fused path
+    double* W0;           // [5][NRp]
+    double* conv;         // [5][NRp]
+    double* diss;         // [5][NRp]
+    double* grad;         // [15][NA]
+    double* lim;          // [5][NA]
+    double* lg;           // [15][NA] limiter * gradient (fused path)
+    double* dt;           // [NRp]
+    double* specrad;      // [NRp]
+    double* part;         // [ntiles][8] per-tile partial sums of the residual monitor
+};
+
+}  // namespace e3d

@@ -1,0 +1,200 @@
+/*
+ * euler3d_amd -- C ABI of the MI355X residual-and-update path of euler3D.
+ *
+ * One e3d_ctx owns the device copy of a multi-block mesh and its flow state,
+ * one HIP stream and the solver parameters. Calls on a context are ordered on
+ * its stream; getters synchronise. Every function returns E3D_OK (0) or a
+ * negative status and never exits or throws; e3d_last_error() gives the text.
+ *
+ * Cell numbering inside a block is the reference's: [0, nreal) real cells,
+ * [nreal, nall) ghost cells in marker order (physical markers first, CONNEXION
+ * last), faces numbered by first appearance. Arrays passed across the ABI are
+ * structure-of-arrays in that numbering: variable-major, e.g. W[v*nall + c]
+ * with v = rho, u, v, w, p; gradients G[(v*3 + d)*nall + c].
+ */
+#ifndef EULER3D_AMD_H
+#define EULER3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define E3D_OK 0
+#define E3D_EINVAL (-1)      /* bad argument or unsupported option */
+#define E3D_ENOMEM (-2)      /* host or device allocation failed */
+#define E3D_EIO (-3)         /* file missing or malformed */
+#define E3D_EHIP (-4)        /* HIP runtime error (no device, launch failure) */
+#define E3D_ESTATE (-5)      /* call out of order (no mesh, no params) */
+#define E3D_ENONFINITE (-6)  /* residual norm became NaN or Inf */
+
+/* Scheme selectors: the reference picks them by string in its factories
+ * (Solver/src/ResidualCalculator.cpp:72-95, Solver/src/SecondOrder.cpp:50-61). */
+#define E3D_FLUX_ROE 0
+#define E3D_FLUX_AUSM 1
+#define E3D_GRAD_GREEN_GAUSS 0
+#define E3D_GRAD_LEAST_SQUARES 1
+#define E3D_LIM_BARTH_JESPERSEN 0
+#define E3D_LIM_VENKATAKRISHNAN 1
+
+/* Array selectors for e3d_get_array / e3d_set_array. */
+#define E3D_ARR_W 0        /* primitive rho,u,v,w,p      5 x nall  */
+#define E3D_ARR_W0 1       /* conservative state at iteration start 5 x nreal */
+#define E3D_ARR_CONV 2     /* convective residual        5 x nreal */
+#define E3D_ARR_DISS 3     /* dissipative residual       5 x nreal */
+#define E3D_ARR_GRAD 4     /* gradients                 15 x nall  */
+#define E3D_ARR_LIM 5      /* limiters                   5 x nall  */
+#define E3D_ARR_DT 6       /* local time step            1 x nreal */
+#define E3D_ARR_SPECRAD 7  /* convective spectral radius 1 x nreal */
+
+typedef struct e3d_ctx e3d_ctx;
+typedef struct e3d_mesh e3d_mesh;
+
+/* Bump channel [0,3]x[0,1]x[0,1], circular-arc bump of thickness h on the
+ * lower wall for 1 <= x <= 2, nx*ny*nz hexes split into 6 tets each.
+ * WALL = lower wall, SYMMETRY = top and sides, FARFIELD = inlet and outlet.
+ * Cells are numbered brick by brick (brick^3 hexes, 0 = lexicographic);
+ * px*py*pz > 1 splits the hexes into that many blocks with CONNEXION markers. */
+typedef struct e3d_bump_desc {
+    int nx, ny, nz;
+    double h;
+    int brick;
+    int px, py, pz;
+} e3d_bump_desc;
+
+/* Read-only view of one built block, in the reference's numbering. Pointers
+ * stay valid until e3d_mesh_free. */
+typedef struct e3d_block_view {
+    int nreal, nall, nface, nnode;
+    int n_boundaries, n_real_boundaries, n_wall_faces;
+    const int *face2cell, *face_nnode, *face_nodes;      /* 2F, F, 4F        */
+    const double *S, *area, *fcenter, *rL, *rR;          /* 3F, F, 3F, 3F, 3F */
+    const int *cell_off, *cell2face, *cell2cell;         /* nall+1, CSR      */
+    const double *volume, *ccenter, *lsqw;               /* nall, 3nall, 3*cell_off[nreal] */
+    const int *bnd_type, *bnd_off, *bnd_ghosts;          /* per marker, CSR of ghost ids */
+    const int *wall_faces;
+    int ghost_count;                                     /* topology GhostCount */
+    int n_interfaces;
+    const int *itf_origin_block, *itf_off, *itf_origin_cells;
+} e3d_block_view;
+
+/* Solver parameters: the numeric content of the reference's input file
+ * (Solver/src/Interface.cpp:33-198) and of its factory choices. */
+typedef struct e3d_params {
+    double gamma, cfl, aoa_deg, mach, cmac;
+    int stages;           /* RK stages 1..5, coefficient tables of Solver/src/RungeKutta.cpp:23-68 */
+    int max_iter;
+    double criterion;     /* stop when log10 density residual drop <= criterion */
+    int smoothing;        /* residual smoothing: only 0 (NoSmoother) is supported */
+    int flux;             /* E3D_FLUX_* */
+    int order;            /* 1 or 2 */
+    int gradient;         /* E3D_GRAD_* */
+    int limiter;          /* E3D_LIM_* */
+    double omega, k;      /* Barth-Jespersen omega, Venkatakrishnan K */
+} e3d_params;
+
+/* One row of the monitor history (PostProcessing/src/PostProcessing.cpp:220-308). */
+typedef struct e3d_monitor {
+    int iter;
+    int stop;             /* 1 when the reference's stop test fires */
+    double rms[5];        /* sum over blocks of (1/N_b) * sqrt(sum_c (conv-diss)^2) */
+    double drop[5];       /* log10(rms) - log10(rms at iteration 0) */
+    double cl, cd, cmx, cmy, cmz;
+} e3d_monitor;
+
+/* ---------------------------------------------------------------- errors */
+const char* e3d_last_error(const e3d_ctx* ctx);   /* ctx may be NULL: last error of this thread */
+const char* e3d_version(void);
+
+/* ------------------------------------------------------------ host mesh  */
+/* Replaces CompleteMesh::InitializeMyBlocks (DataStructure/src/CompleteMesh.cpp:79-293):
+ * reads the topology file and every block file, builds faces and connectivity
+ * (BlockBuilder/src/ConcreteBlockBuilder.cpp:22-742, BlockBuilder/src/BlockBuilder.cpp:147-458)
+ * and geometry (Metrics/src/MetricsInitializer.cpp:27-800). */
+int e3d_mesh_read(const char* topology_path, e3d_mesh** out);
+/* Generates the bump channel in memory and builds it like e3d_mesh_read. */
+int e3d_mesh_bump(const e3d_bump_desc* desc, e3d_mesh** out);
+/* Writes <dir>/<stem>_<b>.su2 per block and <dir>/<stem>.top (reference dialect). */
+int e3d_mesh_write(const e3d_mesh* mesh, const char* dir, const char* stem);
+int e3d_mesh_nblocks(const e3d_mesh* mesh);
+int e3d_mesh_block(const e3d_mesh* mesh, int block, e3d_block_view* out);
+void e3d_mesh_free(e3d_mesh* mesh);
+
+/* Parses the reference's positional input file (Solver/src/Interface.cpp:14-237).
+ * topology_path receives the topology file name (buffer of path_len bytes). */
+int e3d_read_input(const char* input_path, e3d_params* params, char* topology_path, int path_len);
+
+/* ------------------------------------------------------- device context */
+int e3d_create(int device, e3d_ctx** out);
+void e3d_destroy(e3d_ctx* ctx);
+/* Replaces the Solver/RungeKutta/ResidualCalculator constructors
+ * (Solver/src/Solver.cpp:16-22, Solver/src/RungeKutta.cpp:15-72). */
+int e3d_set_params(e3d_ctx* ctx, const e3d_params* params);
+/* Copies every block of the mesh and its interfaces into device memory. */
+int e3d_upload_mesh(e3d_ctx* ctx, const e3d_mesh* mesh);
+int e3d_nblocks(const e3d_ctx* ctx);
+/* PrimitiveVariables::calculateFreeVariables + initializeFlowField
+ * (DataStructure/src/PrimitiveVariables.cpp:41-69): freestream in every cell. */
+int e3d_init_freestream(e3d_ctx* ctx);
+/* Host <-> device copies of one block's arrays (E3D_ARR_*), reference layout. */
+int e3d_get_array(e3d_ctx* ctx, int block, int which, double* out);
+int e3d_set_array(e3d_ctx* ctx, int block, int which, const double* in);
+
+/* ----------------------------------------------- granular operators
+ * block = -1 applies the operator to every block (one launch). */
+/* Solver::saveW0 (Solver/src/Solver.cpp:161-210) */
+int e3d_save_w0(e3d_ctx* ctx, int block);
+/* Timestep::computeSpectralRadius / computeTimestep (Solver/src/Timestep.cpp:26-148) */
+int e3d_spectral_radius(e3d_ctx* ctx, int block);
+int e3d_timestep(e3d_ctx* ctx, int block);
+/* Gradient::computeGradients (Solver/head/Gradient.h:15): GreenGauss.cpp:12-234 or LeastSquares.cpp:12-186 */
+int e3d_gradients(e3d_ctx* ctx, int block);
+/* Limiter::computeLimiters (Solver/head/Limiter.h:15): Venkatakrishnan.cpp:12-326 or BarthJespersen.cpp:14-333 */
+int e3d_limiters(e3d_ctx* ctx, int block);
+/* Interpolation::computeInterpolation (Solver/head/Interpolation.h:15): FirstOrder no-op, SecondOrder gradient+limiter */
+int e3d_interpolation(e3d_ctx* ctx, int block);
+/* FluxScheme::computeFluxConv (Solver/head/FluxScheme.h:20): FluxScheme.cpp:16-211 (Roe) or AusmScheme.cpp:647-871 */
+int e3d_flux_conv(e3d_ctx* ctx, int block);
+/* FluxScheme::computeFluxDiss with current_beta_ (Solver/head/FluxScheme.h:19): RoeScheme.cpp:12-311 or AusmScheme.cpp:12-243 */
+int e3d_flux_diss(e3d_ctx* ctx, int block, double beta);
+/* ResidualSmoother::smoothResidual (Solver/head/ResidualSmoother.h:15): NoSmoother no-op */
+int e3d_smooth_residual(e3d_ctx* ctx, int block);
+/* ResidualCalculator::computeResidual (Solver/src/ResidualCalculator.cpp:24-41) for RK stage `stage` */
+int e3d_residual(e3d_ctx* ctx, int block, int stage);
+/* Updater::updateInternalBlock (Solver/src/Updater.cpp:12-78) for RK stage `stage` */
+int e3d_update(e3d_ctx* ctx, int block, int stage);
+/* Updater::updateBoundary -> BoundaryCellIds::updateBoundary for every physical marker
+ * (Solver/src/Updater.cpp:80-88; WallCellIds.cpp:9-95, SymmetryCellIds.cpp:9-96, FarfieldCellIds.cpp:10-114) */
+int e3d_bc(e3d_ctx* ctx, int block);
+/* RungeKutta::computeRungeKutta (Solver/src/RungeKutta.cpp:80-94) */
+int e3d_runge_kutta(e3d_ctx* ctx, int block);
+/* BlockCommunicator::updateBoundaries (MPI/src/BlockCommunicator.cpp:212-268): interface ghosts
+ * take the primitive state of their origin cells (the only copied field the path reads later). */
+int e3d_exchange(e3d_ctx* ctx);
+/* PostProcessing::computeFlowData + BlockCommunicator::getGlobal + PostProcessing::process
+ * (PostProcessing/src/PostProcessing.cpp:198-308, Convergence.cpp:27-144,
+ * AerodynamicParameters.cpp:163-239): fills one monitor row and advances the iteration count. */
+int e3d_monitors(e3d_ctx* ctx, e3d_monitor* out);
+
+/* ------------------------------------------------------------ fused path
+ * Solver::solve (Solver/src/Solver.cpp:36-98) for up to n iterations with
+ * stage-fused kernels over all blocks; history (may be NULL) receives one row
+ * per iteration. Returns the number of iterations run (>= 0) or a status < 0.
+ * Stops early on the reference's stop test or on e3d_request_stop. */
+int e3d_iterate(e3d_ctx* ctx, int n, e3d_monitor* history);
+/* Equivalent of the reference's STOP file (PostProcessing.cpp:72-80): the next
+ * iteration boundary ends e3d_iterate. */
+int e3d_request_stop(e3d_ctx* ctx);
+/* Resets the iteration counter and the reference residual of the drop. */
+int e3d_reset_monitors(e3d_ctx* ctx);
+
+/* --------------------------------------------------------- measurement */
+/* Average duration in ms of the named kernel over the last e3d_iterate call
+ * (HIP events on the context's stream); kernel = "interp", "flux_update", ... */
+int e3d_kernel_time(e3d_ctx* ctx, const char* kernel, double* avg_ms, int* launches);
+int e3d_enable_timing(e3d_ctx* ctx, int on);
+int e3d_sync(e3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EULER3D_AMD_H */

@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/*.npz from the reference itself.
+
+Container-only: runs oracle/_ref/ref_dump (the reference's own classes, built
+from /root/reference by oracle/Makefile) on bump-channel meshes written in the
+reference's SU2 dialect, and on the reference's own NACA0012 hexahedral mesh,
+then packs the dumped arrays:
+
+  meta            JSON: case parameters, mesh description, dump points
+  g<b>_<name>     block geometry as the reference built it (pins the builder)
+  it<k>_b<b>_<n>  state after iteration k (W over all cells)
+  it<k>s<s>_b<b>_<n>  state right after the residual of stage s of iteration k
+                  (W, gradients, limiters over all cells; conv, diss over real
+                  cells; W0, dt, spectral radius at s = 0)
+  monitor         rows (iter, rms x5, cl, cd, cmx, cmy, cmz, drop ro, drop u, drop p, stop)
+  blockmon_it<i>_b<b>  per-block rms x5 and cl, cd, cm x3 for i < 3
+
+Arrays of up to 512 entries are stored as they are. A larger array X is stored
+as its SHA-256 digest (h:X, of its little-endian bytes) plus the values at 48
+fixed indices (i:X, s:X): parity on this path is bitwise, so the digest is the
+check and the samples locate a mismatch. The NACA0012 mesh itself (a data file
+of the reference) is kept raw so the builder can be run on it without the
+reference tree.
+
+usage: python tests/golden/make_golden.py [--big] [--only NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+REF = "/root/reference"
+DUMPER = os.path.join(REPO, "oracle", "_ref", "ref_dump")
+
+FLUX = {0: "roe", 1: "ausm"}
+GRAD = {0: "greengauss", 1: "leastsquares"}
+LIM = {0: "barthjespersen", 1: "venkatakrishnan"}
+
+# name: mesh, solver parameters, iterations, dump points
+CASES = {
+    "bump1_roe1": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.5, aoa=0.0, flux=0, order=1, grad=0, lim=1,
+                       omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe2_gg_vk": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.675, aoa=0.0, flux=0, order=2, grad=0, lim=1,
+                             omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe2_lsq_bj": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.7, aoa=3.0, flux=0, order=2, grad=1, lim=0,
+                              omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_ausm2_lsq_bj": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.7, aoa=3.0, flux=1, order=2, grad=1, lim=0,
+                               omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_ausm2_gg_vk": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.5, aoa=-2.0, flux=1, order=2, grad=0, lim=1,
+                              omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_ausm1_s3": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.3, aoa=6.0, flux=1, order=1, grad=0, lim=1,
+                           omega=1e-16, k=5.0, stages=3, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe2_gg_bj_s4": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.8, aoa=1.0, flux=0, order=2, grad=0, lim=0,
+                                omega=0.01, k=5.0, stages=4, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe2_lsq_vk_s2": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.6, aoa=2.0, flux=0, order=2, grad=1, lim=1,
+                                 omega=1e-16, k=0.3, stages=2, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe1_s1": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.5, aoa=0.0, flux=0, order=1, grad=0, lim=1,
+                          omega=1e-16, k=5.0, stages=1, iters=8, dumps=[4, 5], stage_iter=5),
+    "bump8_roe2_gg_vk": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=0.675, aoa=0.0, flux=0, order=2, grad=0, lim=1,
+                             omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump8_ausm2_lsq_bj": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=0.7, aoa=3.0, flux=1, order=2, grad=1, lim=0,
+                               omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "naca_hex_roe2_gg_vk": dict(mesh="naca", mach=0.8, aoa=1.25, flux=0, order=2, grad=0, lim=1,
+                                omega=1e-16, k=5.0, stages=5, iters=8, dumps=[4, 5], stage_iter=5),
+}
+BIG = {
+    # config 1 of the benchmark: 48x16x16x6 tets, first-order Roe, M=0.5, 200 iterations
+    "c1_roe1": dict(mesh=(48, 16, 16, (1, 1, 1)), mach=0.5, aoa=0.0, flux=0, order=1, grad=0, lim=1,
+                    omega=1e-16, k=5.0, stages=5, iters=200, dumps=[199], stage_iter=-1),
+}
+
+
+def input_file(path, topology, c):
+    lines = ["MESH", "topologyfilename", topology, "INPUT", "cfl gamma angleattackdeg",
+             f"1.5 1.4 {c['aoa']!r}", "rkstage mach cmac", f"{c['stages']} {c['mach']!r} 1.0",
+             "nbitermax convcriterion", f"{c['iters']} -15", "residual smoothing (0-no 1-yes)", "0",
+             "fluxscheme schemeorder", f"{FLUX[c['flux']]} {c['order']}", "gradient limiter",
+             f"{GRAD[c['grad']]} {LIM[c['lim']]}", "omega k", f"{c['omega']!r} {c['k']!r}",
+             "EXECUTABLE (0-no 1-yes)", "build execute", "1 1", "OUTPUT (0-no 1-yes)",
+             "clalpha cdalpha cmalpha", "1 1 1", "coefficientsconv  residualconv", "1 1",
+             "cpxc machcontour cpcontour machisosurface", "1 1 1 0", "slicecp", "1",
+             "axis xcoord ycoord zcoord", "x 1.00 1.00 1.00"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_su2_arrays(path):
+    """Raw SU2 content as arrays (nodes, elements, markers) for the fixture."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    npoin = int(lines[2].split()[1])
+    xyz = np.array([[float(t) for t in lines[3 + i].split()[:3]] for i in range(npoin)])
+    i = 3 + npoin + 1
+    nelem = int(lines[i].split()[1])
+    el = [list(map(int, lines[i + 1 + e].split())) for e in range(nelem)]
+    i = i + 1 + nelem + 1
+    nmark = int(lines[i].split()[1])
+    i += 1
+    marks = []
+    for _ in range(nmark):
+        tag = lines[i].split()[1]
+        ne = int(lines[i + 1].split()[1])
+        marks.append((tag, [list(map(int, lines[i + 2 + e].split())) for e in range(ne)]))
+        i += 2 + ne
+    nn = {10: 4, 12: 8, 14: 5}  # element lines may carry a trailing index
+    etype = np.array([e[0] for e in el], np.int32)
+    enodes = np.array([n for e in el for n in e[1:1 + nn[e[0]]]], np.int32)
+    mtags = [m[0] for m in marks]
+    mcount = np.array([len(m[1]) for m in marks], np.int32)
+    mtype = np.array([e[0] for m in marks for e in m[1]], np.int32)
+    mnodes = np.array([n for m in marks for e in m[1] for n in e[1:1 + (3 if e[0] == 5 else 4)]], np.int32)
+    return {"su2_xyz": xyz, "su2_etype": etype, "su2_enodes": enodes, "su2_mcount": mcount,
+            "su2_mtype": mtype, "su2_mnodes": mnodes}, mtags
+
+
+RAW_LIMIT = 512
+NSAMPLE = 48
+
+
+def pack(arrays, name, a):
+    if a.size <= RAW_LIMIT or name.startswith("su2_") or name == "meta":
+        arrays[name] = a
+        return
+    import hashlib
+    arrays["h:" + name] = np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), np.uint8)
+    rng = np.random.RandomState(12345)
+    idx = np.sort(rng.choice(a.size, size=min(NSAMPLE, a.size), replace=False)).astype(np.int64)
+    arrays["i:" + name] = idx
+    arrays["s:" + name] = a[idx]
+
+
+def run_case(name, c, outdir):
+    import euler3d_amd as E
+    tmp = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    try:
+        extra = {}
+        meta = dict(c)
+        if c["mesh"] == "naca":
+            top = os.path.join(tmp, "naca.top")
+            with open(top, "w") as f:
+                f.write(f"NBLOCK= 1\n{REF}/Meshes/naca0012_coarse.su2\nBlock= 0\nGhostCount= 0\nNconnexion= 0\n")
+            extra, tags = read_su2_arrays(f"{REF}/Meshes/naca0012_coarse.su2")
+            meta["su2_mtags"] = tags
+        else:
+            nx, ny, nz, parts = c["mesh"]
+            m = E.Mesh.bump(nx, ny, nz, 0.1, 4, parts)
+            top = m.write(tmp, "bump")
+        input_file(os.path.join(tmp, "input"), top, c)
+        dump = os.path.join(tmp, "dump")
+        args = [DUMPER, "input", dump, ",".join(map(str, c["dumps"])), str(c["stage_iter"])]
+        r = subprocess.run(args, cwd=tmp, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: ref_dump failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        arrays = dict(extra)
+        keep_geom = c["mesh"] != (48, 16, 16, (1, 1, 1))
+        for fn in sorted(os.listdir(dump)):
+            base, ext = os.path.splitext(fn)
+            dt = np.int32 if ext == ".i32" else np.float64
+            a = np.fromfile(os.path.join(dump, fn), dtype=dt)
+            if base.startswith("geom_b"):
+                if not keep_geom and not base.endswith(("counts", "freestream")):
+                    continue
+                pack(arrays, "g" + base[len("geom_b"):], a)
+            else:
+                pack(arrays, base, a)
+        meta["monitor_cols"] = ["iter", "rms_ro", "rms_u", "rms_v", "rms_w", "rms_p", "cl", "cd", "cmx",
+                                "cmy", "cmz", "drop_ro", "drop_u", "drop_p", "stop"]
+        meta["mesh"] = list(meta["mesh"]) if meta["mesh"] != "naca" else "naca"
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        path = os.path.join(outdir, name + ".npz")
+        np.savez_compressed(path, **arrays)
+        print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB, {len(arrays)} arrays")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--big", action="store_true", help="also the config-1 case (reference setup takes minutes)")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    if not os.path.exists(DUMPER):
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+    cases = dict(CASES)
+    if a.big:
+        cases.update(BIG)
+    for name, c in cases.items():
+        if a.only and name != a.only:
+            continue
+        run_case(name, c, HERE)
+
+
+if __name__ == "__main__":
+    main()

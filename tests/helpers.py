@@ -1,0 +1,139 @@
+"""Shared test helpers: golden fixtures, digests, mesh building, oracle driving."""
+from __future__ import annotations
+
+import glob
+import hashlib
+import json
+import os
+import tempfile
+
+import numpy as np
+
+import euler3d_amd as E
+import oracle as O
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+SMALL_FIXTURES = [f for f in FIXTURES if not f.startswith("c1_")]
+
+
+class Golden:
+    def __init__(self, name):
+        self.name = name
+        self.z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
+        self.meta = json.loads(bytes(self.z["meta"]).decode())
+
+    def has(self, key):
+        return key in self.z.files or ("h:" + key) in self.z.files
+
+    def check(self, key, mine, what=""):
+        """Bitwise check of `mine` against the reference array `key`."""
+        mine = np.ascontiguousarray(mine)
+        if key in self.z.files:
+            ref = self.z[key]
+            assert mine.shape == ref.shape, f"{self.name}:{key} shape {mine.shape} vs {ref.shape}"
+            if mine.dtype.kind == "f":
+                bad = mine.view(np.int64) != ref.view(np.int64)
+            else:
+                bad = mine != ref
+            assert not bad.any(), f"{self.name}:{key} {what} differs at {np.flatnonzero(bad)[:8]}: " \
+                                  f"{mine[bad][:4]} vs {ref[bad][:4]}"
+            return
+        digest = hashlib.sha256(mine.tobytes()).digest()
+        ref_digest = bytes(self.z["h:" + key])
+        if digest != ref_digest:
+            idx = self.z["i:" + key]
+            samp = self.z["s:" + key]
+            got = mine.reshape(-1)[idx] if mine.size > idx.max() else None
+            diff = None if got is None else np.flatnonzero(got.view(np.int64) != samp.view(np.int64))
+            raise AssertionError(f"{self.name}:{key} {what} digest mismatch (size {mine.size}); sampled diffs at "
+                                 f"{None if diff is None else idx[diff][:8]}: "
+                                 f"{None if diff is None else (got[diff][:4], samp[diff][:4])}")
+
+    def monitor(self):
+        return self.z["monitor"].reshape(-1, 15)
+
+
+def write_su2_from_fixture(g, path):
+    """Writes the raw SU2 arrays of a fixture (the reference's NACA0012 mesh) as a block file."""
+    z, tags = g.z, g.meta["su2_mtags"]
+    xyz, et, en = z["su2_xyz"], z["su2_etype"], z["su2_enodes"]
+    nn = {10: 4, 12: 8, 14: 5}
+    lines = ["NDIME= 3", "", f"NPOIN= {len(xyz)}"]
+    lines += [f"{x!r} {y!r} {zz!r} {i}" for i, (x, y, zz) in enumerate(xyz.tolist())]
+    lines += ["", f"NELEM= {len(et)}"]
+    p = 0
+    for t in et.tolist():
+        lines.append(" ".join(map(str, [t] + en[p:p + nn[t]].tolist())))
+        p += nn[t]
+    lines += ["", f"NMARK= {len(tags)}"]
+    mt, mn, mc = z["su2_mtype"], z["su2_mnodes"], z["su2_mcount"]
+    e = p = 0
+    for tag, cnt in zip(tags, mc.tolist()):
+        lines += [f"MARKER_TAG= {tag}", f"MARKER_ELEMS= {cnt}"]
+        for _ in range(cnt):
+            t = int(mt[e])
+            k = 3 if t == 5 else 4
+            lines.append(" ".join(map(str, [t] + mn[p:p + k].tolist())))
+            p += k
+            e += 1
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def mesh_for(g):
+    """Builds the product mesh of a golden case (same input the reference read)."""
+    m = g.meta["mesh"]
+    if m == "naca":
+        d = tempfile.mkdtemp(prefix="e3d_naca_")
+        su2 = os.path.join(d, "naca.su2")
+        write_su2_from_fixture(g, su2)
+        top = os.path.join(d, "naca.top")
+        with open(top, "w") as f:
+            f.write(f"NBLOCK= 1\n{su2}\nBlock= 0\nGhostCount= 0\nNconnexion= 0\n")
+        return E.Mesh.read(top)
+    nx, ny, nz, parts = m
+    return E.Mesh.bump(nx, ny, nz, 0.1, 4, tuple(parts))
+
+
+def oracle_params(meta):
+    return O.params(meta["mach"], meta["aoa"], flux=meta["flux"], order=meta["order"], gradient=meta["grad"],
+                    limiter=meta["lim"], omega=meta["omega"], k=meta["k"], stages=meta["stages"],
+                    max_iter=meta["iters"], criterion=-15.0)
+
+
+def product_params(meta):
+    return E.make_params(meta["mach"], meta["aoa"], flux=meta["flux"], order=meta["order"],
+                         gradient=meta["grad"], limiter=meta["lim"], omega=meta["omega"], k=meta["k"],
+                         stages=meta["stages"], max_iter=meta["iters"], criterion=-15.0)
+
+
+def geometry_arrays(b):
+    """Product geometry of one block in the dumper's layout."""
+    F = b["nface"]
+    f2n = np.concatenate([b["face_nnode"].reshape(F, 1), b["face_nodes"].reshape(F, 4)], axis=1).reshape(-1)
+    bnd = []
+    for j in range(len(b["bnd_type"])):
+        ids = b["bnd_ghosts"][b["bnd_off"][j]:b["bnd_off"][j + 1]]
+        bnd += [int(b["bnd_type"][j]), len(ids)] + ids.tolist()
+    return {"face2cell": b["face2cell"], "face2node": f2n.astype(np.int32), "S": b["S"], "area": b["area"],
+            "fcenter": b["fcenter"], "rL": b["rL"], "rR": b["rR"], "cell_off": b["cell_off"],
+            "cell2face": b["cell2face"], "cell2cell": b["cell2cell"], "volume": b["volume"],
+            "ccenter": b["ccenter"], "lsqw": b["lsqw"], "boundaries": np.array(bnd, np.int32),
+            "wallfaces": b["wall_faces"]}
+
+
+def real(a, nv, nall, nreal):
+    """Real-cell part of a variable-major all-cell array."""
+    return np.ascontiguousarray(a.reshape(nv, nall)[:, :nreal]).reshape(-1)
+
+
+def oracle_state(B, what):
+    na, nr = B.nall, B.nreal
+    return {"W": B.W, "grad": B.grad, "lim": B.lim, "conv": real(B.conv, 5, na, nr), "diss": real(B.diss, 5, na, nr),
+            "W0": real(B.W0, 5, na, nr), "dt": B.dt[:nr], "specrad": B.specrad[:nr]}[what]
+
+
+def values_equal(a, b):
+    """Elementwise equality treating +0 == -0 and NaN == NaN."""
+    return (a == b) | (np.isnan(a) & np.isnan(b))
