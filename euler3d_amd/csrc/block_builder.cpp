@@ -18,6 +18,7 @@
 // order of the reference statement it restates.
 #include <algorithm>
 #include <climits>
+#include <string>
 #include <cmath>
 #include <stdexcept>
 #include <unordered_map>
@@ -135,18 +136,16 @@ HostBlock build_block(const RawBlock& raw) {
         if (occ_count[f] >= 2) throw std::runtime_error("mesh: boundary element on an interior face");
         occ_cells[2 * f + occ_count[f]++] = gc;
     }
-    // face -> cells: ascending cell index; a face with one cell keeps 0 as its
-    // second entry, as the reference's zero-initialised array does.
+    // face -> cells: ascending cell index. A face with a single cell (a boundary
+    // face without a marker element) would get cell 0 as its right cell in the
+    // reference (zero-initialised array); it is rejected here instead.
     B.face2cell.resize(2 * (size_t)B.nface);
     for (int f = 0; f < B.nface; ++f) {
-        int a = occ_cells[2 * f], b = occ_cells[2 * f + 1];
-        if (occ_count[f] == 1) {
-            B.face2cell[2 * f] = a;
-            B.face2cell[2 * f + 1] = 0;
-        } else {
-            B.face2cell[2 * f] = std::min(a, b);
-            B.face2cell[2 * f + 1] = std::max(a, b);
-        }
+        if (occ_count[f] == 1)
+            throw std::runtime_error("mesh: face " + std::to_string(f) + " has unmatched faces (open boundary: no marker element)");
+        const int a = occ_cells[2 * f], b = occ_cells[2 * f + 1];
+        B.face2cell[2 * f] = std::min(a, b);
+        B.face2cell[2 * f + 1] = std::max(a, b);
     }
 
     // cell -> faces and cell -> cells in ascending face order.

@@ -39,8 +39,11 @@ void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, cons
 using namespace e3d;
 
 namespace {
-enum KernelId { KI_INTERP, KI_PREP, KI_FLUX_UPDATE, KI_EXCHANGE, KI_MONITORS, KI_COUNT };
-const char* kKernelNames[KI_COUNT] = {"interp", "prep", "flux_update", "exchange", "monitors"};
+enum KernelId { KI_INTERP0, KI_INTERP, KI_PREP, KI_FLUX_UPDATE, KI_FLUX_UPDATE_ND, KI_EXCHANGE, KI_MONITORS, KI_COUNT };
+// interp0 / interp: stage-0 (with W0 and time step) and later stages; flux_update /
+// flux_update_nodiss: stages with beta != 0 and beta == 0.
+const char* kKernelNames[KI_COUNT] = {"interp0", "interp", "prep", "flux_update", "flux_update_nodiss",
+                                      "exchange", "monitors"};
 }  // namespace
 
 struct e3d_ctx {
@@ -80,8 +83,8 @@ struct e3d_ctx {
     };
     std::vector<Ev> ev_pool;
     size_t ev_used = 0;
-    double t_ms[KI_COUNT] = {0, 0, 0, 0, 0};
-    int t_n[KI_COUNT] = {0, 0, 0, 0, 0};
+    double t_ms[KI_COUNT] = {};
+    int t_n[KI_COUNT] = {};
 };
 
 namespace e3d {
@@ -463,14 +466,11 @@ int e3d_nblocks(const e3d_ctx* c) { return c ? c->nb : E3D_EINVAL; }
 
 int e3d_init_freestream(e3d_ctx* c) {
     if (!c || !c->have_mesh || !c->have_params) return set_error(c, E3D_ESTATE, "e3d_init_freestream: need mesh and params");
-    std::vector<double> w(c->m.NA);
-    for (int v = 0; v < 5; ++v) {
-        std::fill(w.begin(), w.end(), c->fs[v]);
-        HIP_TRY(c, hipMemcpyAsync(c->s.W + (size_t)v * c->m.NA, w.data(), sizeof(double) * c->m.NA,
-                                  hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(c->s.Wn + (size_t)v * c->m.NA, w.data(), sizeof(double) * c->m.NA,
-                                  hipMemcpyHostToDevice, c->stream));
-    }
+    const size_t NA = c->m.NA;
+    std::vector<double> w(5 * NA);  // one host buffer per variable, alive until the copies complete
+    for (int v = 0; v < 5; ++v) std::fill(w.begin() + v * NA, w.begin() + (v + 1) * NA, c->fs[v]);
+    HIP_TRY(c, hipMemcpyAsync(c->s.W, w.data(), sizeof(double) * 5 * NA, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->s.Wn, w.data(), sizeof(double) * 5 * NA, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->iter = 0;
     c->started = false;
@@ -681,7 +681,7 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
         for (int st = 0; st < p.stages; ++st) {
             size_t ev = 0;
             if (order2) {
-                ev_begin(c, KI_INTERP, &ev);
+                ev_begin(c, st == 0 ? KI_INTERP0 : KI_INTERP, &ev);
                 launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, st == 0, k * k * k,
                                     p.omega, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
                 ev_end(c, ev);
@@ -690,7 +690,7 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
                 launch_prep_fused(c->m, c->s, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
                 ev_end(c, ev);
             }
-            ev_begin(c, KI_FLUX_UPDATE, &ev);
+            ev_begin(c, c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
             launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, c->gas, c->beta[st],
                                c->alpha[st], c->fs, c->stream);
             ev_end(c, ev);

@@ -143,7 +143,9 @@ def freestream(p):
 class Solver:
     """Multi-block oracle solver: Solver::solve semantics on OracleBlocks."""
 
-    def __init__(self, blocks_geom, p):
+    def __init__(self, blocks_geom, p, interfaces=True):
+        """interfaces=False drops the block interfaces (their ghosts keep their
+        values): used to time one block of a multi-block mesh on its own."""
         self.blocks = [OracleBlock(g) for g in blocks_geom]
         self.p = p
         self.fs = freestream(p)
@@ -151,7 +153,7 @@ class Solver:
         # re-point the array copies at the same state buffers (structs are copied by value)
         self._itf_keep = []
         itfs = []
-        for d, g in enumerate(blocks_geom):
+        for d, g in enumerate(blocks_geom if interfaces else []):
             # destination ghosts are numbered from GhostCount in topology order
             # (DataStructure/src/CompleteMesh.cpp:240-250)
             off = np.asarray(g["itf_off"])

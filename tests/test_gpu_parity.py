@@ -1,0 +1,213 @@
+"""GPU parity: the HIP path through the C ABI against the reference and the oracle.
+
+* Granular operators (one C-ABI call per reference operator, driven in the
+  reference's Solver::solve order) against the reference's golden digests,
+  bit for bit: gradients, limiters, residuals, W0, dt, spectral radius after
+  every stage residual of iteration 5, and the state after iterations 4 and 5.
+* The fused path (e3d_iterate: two kernels per RK stage) against the oracle:
+  every cell's state equal as values (+0 == -0; beta = 0 stages skip the
+  dissipation, which only moves the sign of a zero), monitors within 1e-12
+  relative (different summation order of the norms and force sums).
+"""
+import numpy as np
+import pytest
+
+import euler3d_amd as E
+import oracle as O
+from helpers import SMALL_FIXTURES, Golden, mesh_for, oracle_params, product_params, values_equal
+
+pytestmark = pytest.mark.gpu
+
+ARR = {"W": E.ARR_W, "grad": E.ARR_GRAD, "lim": E.ARR_LIM, "conv": E.ARR_CONV, "diss": E.ARR_DISS,
+       "W0": E.ARR_W0, "dt": E.ARR_DT, "specrad": E.ARR_SPECRAD}
+MON_RTOL = 1e-12
+
+
+def gpu_context(mesh, params):
+    ctx = E.Context(0)
+    ctx.set_params(params)
+    ctx.upload(mesh)
+    ctx.init_freestream()
+    return ctx
+
+
+def close_rel(a, b, rtol=MON_RTOL):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-300)) or np.allclose(a, b, rtol=rtol, atol=1e-280)
+
+
+@pytest.mark.parametrize("case", SMALL_FIXTURES)
+def test_granular_operators_match_reference_bitwise(case):
+    g = Golden(case)
+    meta = g.meta
+    mesh = mesh_for(g)
+    nb = mesh.nblocks
+    ctx = gpu_context(mesh, product_params(meta))
+    k = meta["stage_iter"]
+    ctx.op("exchange")                      # Solver.cpp:48
+    ref_mon = g.monitor()
+    for it in range(max(meta["dumps"]) + 1):
+        for b in range(nb):                 # Solver.cpp:83-91
+            ctx.op("save_w0", b)
+            ctx.op("spectral_radius", b)
+            ctx.op("timestep", b)
+            for s in range(meta["stages"]):
+                ctx.op("residual", b, s)
+                if it == k:
+                    tag = f"it{it}s{s}_b{b}_"
+                    for what in ("W", "grad", "lim", "conv", "diss") + (("W0", "dt", "specrad") if s == 0 else ()):
+                        g.check(tag + what, ctx.get(b, ARR[what]).reshape(-1), f"GPU stage {s}")
+                ctx.op("update", b, s)
+                ctx.op("bc", b)
+        ctx.op("exchange")                  # Solver.cpp:94
+        m = ctx.monitors()
+        r = ref_mon[it]
+        assert close_rel(m["rms"], r[1:6]), (it, m["rms"], r[1:6])
+        assert close_rel([m["cl"], m["cd"], m["cmx"], m["cmy"], m["cmz"]], r[6:11]), it
+        if it in meta["dumps"]:
+            for b in range(nb):
+                g.check(f"it{it}_b{b}_W", ctx.get(b, E.ARR_W).reshape(-1), f"GPU after iteration {it}")
+    ctx.close()
+
+
+def run_both(mesh, meta, n):
+    ctx = gpu_context(mesh, product_params(meta))
+    hist = ctx.iterate(n)
+    geoms = [mesh.block(b) for b in range(mesh.nblocks)]
+    S = O.Solver(geoms, oracle_params(meta))
+    S.init_freestream()
+    rh = S.iterate(n)
+    return ctx, hist, S, rh
+
+
+def assert_states_equal(ctx, S):
+    for b, B in enumerate(S.blocks):
+        W = ctx.get(b, E.ARR_W).reshape(-1)
+        same = values_equal(W, B.W)
+        assert same.all(), f"block {b}: {np.sum(~same)} cells differ, first {np.flatnonzero(~same)[:5]}"
+        d = ctx.get(b, E.ARR_DISS).reshape(-1)
+        dref = B.diss.reshape(5, B.nall)[:, :B.nreal].reshape(-1)
+        assert values_equal(d, dref).all(), f"block {b}: dissipation differs"
+
+
+def assert_monitors_close(hist, rh):
+    assert len(hist) == len(rh)
+    for a, r in zip(hist, rh):
+        assert a["iter"] == r["iter"] and a["stop"] == r["stop"]
+        assert close_rel(a["rms"], r["rms"]), (a["iter"], a["rms"], r["rms"])
+        assert close_rel([a["cl"], a["cd"], a["cmx"], a["cmy"], a["cmz"]],
+                         [r["cl"], r["cd"], r["cmx"], r["cmy"], r["cmz"]]), a["iter"]
+
+
+@pytest.mark.parametrize("case", SMALL_FIXTURES)
+def test_fused_iterations_match_oracle(case):
+    g = Golden(case)
+    mesh = mesh_for(g)
+    ctx, hist, S, rh = run_both(mesh, g.meta, g.meta["iters"])
+    assert_states_equal(ctx, S)
+    assert_monitors_close(hist, rh)
+    # and against the reference's own monitor history
+    ref = g.monitor()
+    for a, r in zip(hist, ref):
+        assert close_rel(a["rms"], r[1:6]) and close_rel([a["cl"], a["cd"]], r[6:8])
+    ctx.close()
+
+
+@pytest.mark.parametrize("scheme", ["roe1", "roe2_gg_vk", "ausm2_lsq_bj", "roe2_lsq_bj_8b"])
+def test_fused_medium_mesh_matches_oracle(scheme):
+    """24x8x8 hexes (9,216 tets), several tiles per block, against the oracle."""
+    meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=8)
+    parts = (1, 1, 1)
+    if scheme == "roe1":
+        meta.update(order=1, mach=0.5, aoa=0.0)
+    elif scheme == "ausm2_lsq_bj":
+        meta.update(flux=1, grad=1, lim=0, mach=0.7, aoa=3.0)
+    elif scheme == "roe2_lsq_bj_8b":
+        meta.update(grad=1, lim=0)
+        parts = (2, 2, 2)
+    mesh = E.Mesh.bump(24, 8, 8, 0.1, 4, parts)
+    ctx, hist, S, rh = run_both(mesh, meta, meta["iters"])
+    assert_states_equal(ctx, S)
+    assert_monitors_close(hist, rh)
+    ctx.close()
+
+
+def test_fused_is_deterministic_and_resumable():
+    """Two runs give identical bits; 3 + 3 iterations equal 6 in one call."""
+    meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=50)
+    mesh = E.Mesh.bump(16, 6, 6, 0.1, 4, (2, 1, 1))
+    a = gpu_context(mesh, product_params(meta))
+    ha = a.iterate(6)
+    b = gpu_context(mesh, product_params(meta))
+    hb = b.iterate(3) + b.iterate(3)
+    for blk in range(mesh.nblocks):
+        wa, wb = a.get(blk, E.ARR_W), b.get(blk, E.ARR_W)
+        assert (wa.view(np.int64) == wb.view(np.int64)).all()
+    assert [h["rms"] for h in ha] == [h["rms"] for h in hb]
+    a.close()
+    b.close()
+
+
+def test_fused_equals_granular_path():
+    """e3d_iterate and the per-operator ABI calls produce the same state."""
+    meta = dict(mach=0.7, aoa=3.0, flux=1, order=2, grad=1, lim=0, omega=1e-16, k=5.0, stages=5, iters=50)
+    mesh = E.Mesh.bump(12, 4, 4, 0.1, 4, (2, 1, 1))
+    a = gpu_context(mesh, product_params(meta))
+    a.iterate(3)
+    b = gpu_context(mesh, product_params(meta))
+    b.op("exchange")
+    for _ in range(3):
+        for blk in range(mesh.nblocks):
+            b.op("save_w0", blk)
+            b.op("spectral_radius", blk)
+            b.op("timestep", blk)
+            b.op("runge_kutta", blk)
+        b.op("exchange")
+        b.monitors()
+    for blk in range(mesh.nblocks):
+        assert values_equal(a.get(blk, E.ARR_W), b.get(blk, E.ARR_W)).all()
+    a.close()
+    b.close()
+
+
+def test_stop_criterion_and_request():
+    meta = dict(mach=0.5, aoa=0.0, flux=0, order=1, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=40)
+    mesh = E.Mesh.bump(8, 3, 3, 0.1, 4)
+    ctx = gpu_context(mesh, product_params(meta))
+    h = ctx.iterate(100)
+    assert len(h) == 40 and h[-1]["stop"] == 1 and all(x["stop"] == 0 for x in h[:-1])  # max_iter
+    p = product_params(meta)
+    p.criterion = -0.2
+    ctx2 = gpu_context(mesh, p)
+    h2 = ctx2.iterate(100)
+    assert h2[-1]["drop"][0] <= -0.2 and all(x["drop"][0] > -0.2 for x in h2[:-1])
+    ctx3 = gpu_context(mesh, product_params(meta))
+    ctx3.iterate(2)
+    E.lib().e3d_request_stop(ctx3._h)
+    assert ctx3.iterate(5) == []
+    for c in (ctx, ctx2, ctx3):
+        c.close()
+
+
+def test_abi_errors_on_device():
+    ctx = E.Context(0)
+    with pytest.raises(E.E3DError) as e:
+        ctx.op("residual", 0, 0)
+    assert e.value.code == -5  # no mesh
+    p = E.make_params(0.5)
+    p.stages = 7
+    with pytest.raises(E.E3DError) as e:
+        ctx.set_params(p)
+    assert e.value.code == -1
+    p = E.make_params(0.5)
+    p.smoothing = 1
+    with pytest.raises(E.E3DError):
+        ctx.set_params(p)
+    ctx.set_params(E.make_params(0.5))
+    ctx.upload(E.Mesh.bump(4, 2, 2, 0.1, 4))
+    with pytest.raises(E.E3DError) as e:
+        ctx.op("update", 0, 9)
+    assert e.value.code == -1
+    with pytest.raises(E.E3DError):
+        ctx.op("bc", 3)
+    ctx.close()
