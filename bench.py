@@ -32,17 +32,19 @@ MODEL_BYTES_PER_CELL_STAGE = 1110.6  # compulsory-traffic model of this path at 
 
 
 def algorithmic_bytes(blocks):
-    """Compulsory bytes of one flux_update launch (stage with dissipation).
+    """Algorithmic bytes of one flux_update launch (a stage with dissipation).
 
-    Each array element counted once per launch: W and lim*grad of every real and
-    ghost cell (40 + 120 B), face S, rL, rR (72 B/face), per-cell topology
-    (nf 4 B + 4 slots x (neighbour 4 + face 4) B), W0 40, dt 8, volume 8,
-    dissipation read + write 80, new W 40 per real cell and new W 40 per ghost."""
+    The compulsory-traffic model of this path: every array element counted once
+    per kernel that must read or write it (W 40 B/cell, grad 120, lim 40, W0 40,
+    conv+diss 80, dt 8, V 8; per face S 24 + rL, rR 48 + left/right 8 = 80 B).
+    flux_update does the residual kernel K_B = 200(N+G) + 80F + 120N and the
+    update + BC kernel K_C = 176N + 112G of that model in one launch."""
     N = sum(b["nreal"] for b in blocks)
     G = sum(b["nall"] - b["nreal"] for b in blocks)
     F = sum(b["nface"] for b in blocks)
-    base = 160 * (N + G) + 72 * F + 36 * N + 80 * N + 96 * N + 40 * G
-    return base, N, G, F
+    kb = 200 * (N + G) + 80 * F + 120 * N
+    kc = 176 * N + 112 * G
+    return kb + kc, N, G, F
 
 
 def main():
@@ -56,6 +58,9 @@ def main():
     ap.add_argument("--parts", type=int, nargs=3, default=[2, 2, 2])
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, one 1 GiB device-to-device copy: the reference kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the multi-GPU "
@@ -93,8 +98,7 @@ def main():
         ms, n = ctx.kernel_time(name)
         kt[name] = {"avg_ms": ms, "launches": n}
     fu = kt["flux_update"]
-    # of the 3 dissipative stages per iteration, the last also stores the convective residual
-    bytes_per_launch = bytes_fu + 40 * N / 3.0
+    bytes_per_launch = bytes_fu
     achieved = bytes_per_launch / (fu["avg_ms"] * 1e-3) / 1e9 if fu["avg_ms"] > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "flux_update_c2_traffic.json")
@@ -105,6 +109,17 @@ def main():
     print(f"[bench] {a.steps} steps in {wall:.3f}s; rms_ro {rms_ro:.6e}; kernels "
           + ", ".join(f"{k} {v['avg_ms']:.3f}ms x{v['launches']}" for k, v in kt.items()), file=sys.stderr, flush=True)
     ctx.close()
+    if a.calib_copy:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library already loaded
+        n = ctypes.c_size_t(1 << 30)
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        ok = hip.hipMalloc(ctypes.byref(src), n) == 0 and hip.hipMalloc(ctypes.byref(dst), n) == 0
+        ok = ok and hip.hipMemset(src, 1, n) == 0 and hip.hipMemcpy(dst, src, n, 3) == 0
+        ok = ok and hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        print(f"[bench] calibration copy of 1 GiB: {'done' if ok else 'FAILED'}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:

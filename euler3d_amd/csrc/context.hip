@@ -358,6 +358,10 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     std::vector<double> vol(NA, 1.0), lsqw((size_t)nslot * 3 * NR, 0.0);
     std::vector<int> wg_block(NR / kWG, 0), tile0(nb), ntile(nb), wall_off(nb + 1, 0), wall_ghost, wall_face;
     std::vector<int> itf_dst, itf_org;
+    // per-slot face geometry of the fused path
+    const size_t NS = (size_t)nslot * NR;
+    std::vector<double> sS(3 * NS, 0.0), sR(3 * NS, 0.0), sN(3 * NS, 0.0), sA(NS, 1.0), g_r(3 * (size_t)NG, 0.0);
+    std::vector<int> mate(NS, 0);
     for (int b = 0; b < nb; ++b) {
         const HostBlock& B = mesh->built[b];
         const auto& F = mesh->flat[b];
@@ -381,10 +385,36 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
             for (int k = 0; k < nf[gc]; ++k) {
                 const int f = B.cell2face[B.cell_off[q] + k];
                 const bool right = B.face2cell[2 * f] != q;
-                nbr[(size_t)k * NR + gc] = global_cell(c, b, B.cell2cell[B.cell_off[q] + k]);
+                const int other = B.cell2cell[B.cell_off[q] + k];
+                nbr[(size_t)k * NR + gc] = global_cell(c, b, other);
                 fid[(size_t)k * NR + gc] = (c->foff[b] + f) | (right ? (int)0x80000000u : 0);
                 for (int d = 0; d < 3; ++d)
                     lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
+                // geometry copies: S, own-side r, S/|S| and |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz)
+                // as in FluxScheme.cpp:105-112 (equal to the stored face area)
+                const double sx = B.S[3 * f], sy = B.S[3 * f + 1], sz = B.S[3 * f + 2];
+                const double norm = std::sqrt(sx * sx + sy * sy + sz * sz);
+                const double* rv = right ? &B.rR[3 * f] : &B.rL[3 * f];
+                const double sv[3] = {sx, sy, sz};
+                for (int d = 0; d < 3; ++d) {
+                    sS[((size_t)d * nslot + k) * NR + gc] = sv[d];
+                    sR[((size_t)d * nslot + k) * NR + gc] = rv[d];
+                    sN[((size_t)d * nslot + k) * NR + gc] = sv[d] / norm;
+                }
+                sA[(size_t)k * NR + gc] = norm;
+                // face-state record of the other cell on this face
+                size_t rec;
+                if (other < B.nreal) {
+                    int ko = -1;
+                    for (int j = B.cell_off[other]; j < B.cell_off[other + 1]; ++j)
+                        if (B.cell2face[j] == f) ko = j - B.cell_off[other];
+                    if (ko < 0) return set_error(c, E3D_EINVAL, "upload: inconsistent face connectivity");
+                    rec = (size_t)ko * NR + (size_t)(c->roff[b] + other);
+                } else {
+                    rec = ghost_record(nslot, (int)NR, c->goff[b] + (other - B.nreal));
+                }
+                if (rec >= 0x80000000ull) return set_error(c, E3D_EINVAL, "mesh too large for face-state records");
+                mate[(size_t)k * NR + gc] = (int)rec | (right ? (int)0x80000000u : 0);
             }
         }
         for (size_t j = 0; j < B.bnds.size(); ++j) {
@@ -394,6 +424,7 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
                 const int gi = c->goff[b] + (gl - B.nreal);
                 g_in[gi] = global_cell(c, b, B.cell2cell[B.cell_off[gl]]);
                 g_face[gi] = c->foff[b] + B.cell2face[B.cell_off[gl]];
+                for (int d = 0; d < 3; ++d) g_r[(size_t)d * NG + gi] = B.rR[3 * (size_t)B.cell2face[B.cell_off[gl]] + d];
                 int kind = 0;
                 if (real_bnd) {
                     kind |= GK_REFLECT;
@@ -424,8 +455,8 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     int rc = 0;
     const int* cnf;
     (void)cnf;
-    int *d_nf, *d_nbr, *d_fid, *d_gin, *d_gface, *d_gkind, *d_wgb;
-    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw;
+    int *d_nf, *d_nbr, *d_fid, *d_gin, *d_gface, *d_gkind, *d_wgb, *d_mate;
+    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw, *d_sS, *d_sR, *d_sN, *d_sA, *d_gr;
     if ((rc = upload(c, &d_nf, nf)) || (rc = upload(c, &d_nbr, nbr)) || (rc = upload(c, &d_fid, fid)) ||
         (rc = upload(c, &d_gin, g_in)) || (rc = upload(c, &d_gface, g_face)) || (rc = upload(c, &d_gkind, g_kind)) ||
         (rc = upload(c, &d_wgb, wg_block)) || (rc = upload(c, &d_S, S)) || (rc = upload(c, &d_area, area)) ||
@@ -434,16 +465,19 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
         (rc = upload(c, &c->d_itf_org, itf_org)) || (rc = upload(c, &c->d_tile0, tile0)) ||
         (rc = upload(c, &c->d_ntile, ntile)) || (rc = upload(c, &c->d_nreal, c->nreal)) ||
         (rc = upload(c, &c->d_wall_off, wall_off)) || (rc = upload(c, &c->d_wall_ghost, wall_ghost)) ||
-        (rc = upload(c, &c->d_wall_face, wall_face))) {
+        (rc = upload(c, &c->d_wall_face, wall_face)) || (rc = upload(c, &d_mate, mate)) ||
+        (rc = upload(c, &d_sS, sS)) || (rc = upload(c, &d_sR, sR)) || (rc = upload(c, &d_sN, sN)) ||
+        (rc = upload(c, &d_sA, sA)) || (rc = upload(c, &d_gr, g_r))) {
         free_device(c);
         return rc;
     }
     m.nf = d_nf; m.nbr = d_nbr; m.fid = d_fid; m.S = d_S; m.area = d_area; m.rL = d_rL; m.rR = d_rR;
     m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
+    m.sS = d_sS; m.sR = d_sR; m.sN = d_sN; m.sA = d_sA; m.mate = d_mate; m.g_r = d_gr;
     DState& s = c->s;
     if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
         (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
-        (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.lg, 15 * NA)) || (rc = dalloc(c, &s.dt, NR)) ||
+        (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.Wf, 5 * (NS + NG))) || (rc = dalloc(c, &s.dt, NR)) ||
         (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / kWG + 1))) ||
         (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb))) {
         free_device(c);

@@ -34,7 +34,21 @@ struct DMesh {
     const int* g_face;    // [NG] face of each ghost
     const int* g_kind;    // [NG] GK_* bits
     const int* wg_block;  // [NRp/kWG] block of each workgroup-sized tile
+    // Per-slot copies of face geometry for the fused path, slot-major so a
+    // workgroup streams them coalesced: component q of slot k of cell c at
+    // (q*nslot + k)*NRp + c.
+    const double* sS;     // [3][nslot][NRp] area vector S of the face (its own orientation, left -> right)
+    const double* sR;     // [3][nslot][NRp] r vector of the face on this cell's side
+    const double* sN;     // [3][nslot][NRp] unit normal S/|S|
+    const double* sA;     // [nslot][NRp] |S|
+    const int* mate;      // [nslot][NRp] face-state record of the other cell of the face; bit 31: this cell is right
+    const double* g_r;    // [3][NG] r vector of each ghost on its face (the reflected right vector)
 };
+
+// Face-state records of the fused path: record k*NRp + c holds the
+// reconstructed primitive state of real cell c on its k-th face, record
+// nslot*NRp + i that of ghost i on its face. Five doubles per record.
+__device__ __host__ inline size_t ghost_record(int nslot, int NRp, int i) { return (size_t)nslot * NRp + i; }
 
 // Cell range of a granular launch: real cells [c0, c1) (global), ghosts
 // [g0, g1) (ghost index, global cell NRp + i). One block or all blocks.
@@ -50,7 +64,7 @@ struct DState {
     double* diss;         // [5][NRp]
     double* grad;         // [15][NA]
     double* lim;          // [5][NA]
-    double* lg;           // [15][NA] limiter * gradient (fused path)
+    double* Wf;           // [nslot*NRp + NG][5] reconstructed face states (fused path)
     double* dt;           // [NRp]
     double* specrad;      // [NRp]
     double* part;         // [ntiles][8] per-tile partial sums of the residual monitor

@@ -19,43 +19,10 @@
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
+#include "kcommon.hpp"
 #include "physics.hpp"
 
 namespace e3d {
-
-// Tile of kWG cells handled by this workgroup. Workgroups are dispatched
-// round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so that
-// neighbour gathers of adjacent tiles hit the same L2.
-__device__ __forceinline__ int xcd_tile(int ntiles) {
-    const int b = blockIdx.x;
-    const int per = (ntiles + 7) >> 3;
-    return (b & 7) * per + (b >> 3);
-}
-static inline int xcd_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
-
-struct Slot {
-    int nb, f;
-    bool right;
-};
-__device__ __forceinline__ Slot slot(const DMesh& m, int k, int c) {
-    Slot s;
-    s.nb = m.nbr[(size_t)k * m.NRp + c];
-    const int fw = m.fid[(size_t)k * m.NRp + c];
-    s.right = fw < 0;
-    s.f = fw & 0x7fffffff;
-    return s;
-}
-
-__device__ __forceinline__ void load_w(const double* W, int NA, int x, double w[5]) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) w[v] = W[(size_t)v * NA + x];
-}
-
-__device__ __forceinline__ void load3(const double* a, int N, int i, double r[3]) {
-    r[0] = a[i];
-    r[1] = a[(size_t)N + i];
-    r[2] = a[2 * (size_t)N + i];
-}
 
 // Reconstructed face state W + (lim*grad).r (FluxScheme.cpp:127-134), from the
 // separate limiter and gradient arrays (granular path).
@@ -356,10 +323,6 @@ __global__ void __launch_bounds__(kWG) k_update(Range rg, DMesh m, DState s, dou
     for (int v = 0; v < 5; ++v) s.W[(size_t)v * m.NA + c] = w[v];
 }
 
-struct Free {
-    double v[5];
-};
-
 __device__ __forceinline__ void apply_bc(const DMesh& m, int kind, int f, double gamma, const Free& fs,
                                          const double wi[5], double g[5]) {
     const double area = m.area[f];
@@ -400,207 +363,6 @@ __global__ void __launch_bounds__(kWG) k_exchange(int n, const int* dst, const i
         for (int v = 0; v < 5; ++v) lim[(size_t)v * NA + d] = lim[(size_t)v * NA + o];
 #pragma unroll
         for (int q = 0; q < 15; ++q) grad[(size_t)q * NA + d] = grad[(size_t)q * NA + o];
-    }
-}
-
-// ------------------------------------------------------------------- fused
-
-// Stage kernel 1: gradient + limiter of every real cell and of the ghosts
-// hanging off it; writes lim*grad, the only form the flux needs. On stage 0 it
-// also does saveW0 and the spectral radius / time step from the same loads.
-template <int GRAD, bool VENKAT, bool FIRST>
-__global__ void __launch_bounds__(kWG) k_interp_fused(DMesh m, DState s, double k3, double omega, double gamma,
-                                                      double inv_gm1, double cfl, int ntiles) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
-    if (nf == 0) return;
-    double wc[5], g[15];
-    load_w(s.W, m.NA, c, wc);
-    cell_gradient<GRAD>(m, s.W, c, nf, wc, g);
-    const double volc = m.vol[c];
-    const double eps2 = k3 * (volc);
-    LimAcc<VENKAT> L;
-    L.start(wc);
-    for (int k = 0; k < nf; ++k) {
-        double wn[5];
-        load_w(s.W, m.NA, slot(m, k, c).nb, wn);
-        L.neighbour(wn);
-    }
-    for (int k = 0; k < nf; ++k) {
-        const Slot sl = slot(m, k, c);
-        double r[3];
-        load3(sl.right ? m.rR : m.rL, m.NF, sl.f, r);
-        L.face(g, r, eps2, omega);
-        if (sl.nb >= m.NRp) {  // ghost hanging off this cell: its gradient and limiter
-            const int gi = sl.nb - m.NRp;
-            double gg[15], wg[5], rr[3];
-            ghost_gradient(m, m.g_kind[gi], sl.f, g, gg);
-            load_w(s.W, m.NA, sl.nb, wg);
-            LimAcc<VENKAT> LG;
-            LG.start(wg);
-            LG.neighbour(wc);
-            load3(m.rR, m.NF, sl.f, rr);
-            LG.face(gg, rr, eps2, omega);  // ghost volume equals the interior's
-#pragma unroll
-            for (int v = 0; v < 5; ++v)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) s.lg[(size_t)(v * 3 + d) * m.NA + sl.nb] = LG.lim[v] * gg[v * 3 + d];
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) s.lg[(size_t)(v * 3 + d) * m.NA + c] = L.lim[v] * g[v * 3 + d];
-    if (FIRST) {
-        const size_t N = m.NRp;
-        s.W0[c] = wc[0];
-        s.W0[N + c] = wc[0] * wc[1];
-        s.W0[2 * N + c] = wc[0] * wc[2];
-        s.W0[3 * N + c] = wc[0] * wc[3];
-        s.W0[4 * N + c] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
-        s.dt[c] = cfl * volc / spectral_radius(m, c, nf, wc, gamma);
-    }
-}
-
-// Stage 0 of a first-order run: saveW0 + spectral radius + time step.
-__global__ void __launch_bounds__(kWG) k_prep_fused(DMesh m, DState s, double gamma, double inv_gm1, double cfl,
-                                                    int ntiles) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
-    if (nf == 0) return;
-    double wc[5];
-    load_w(s.W, m.NA, c, wc);
-    const size_t N = m.NRp;
-    s.W0[c] = wc[0];
-    s.W0[N + c] = wc[0] * wc[1];
-    s.W0[2 * N + c] = wc[0] * wc[2];
-    s.W0[3 * N + c] = wc[0] * wc[3];
-    s.W0[4 * N + c] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
-    s.dt[c] = cfl * m.vol[c] / spectral_radius(m, c, nf, wc, gamma);
-}
-
-// Reconstructed state from the fused path's limited gradient. First order keeps
-// the reference's arithmetic with its zero gradient and limiter (+0 * r).
-template <bool ORDER2>
-__device__ __forceinline__ void recon_fused(const DState& s, int NA, int x, const double w[5], const double r[3],
-                                            double out[5]) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = w[v];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            const double lg = ORDER2 ? s.lg[(size_t)(v * 3 + d) * NA + x] : 0.0;
-            out[v] += lg * r[d];
-        }
-}
-
-// Stage kernel 2: residual (conv + beta-blended diss) + RK update + ghost BCs.
-// Reads W, writes Wn (double buffer, so neighbours never see a half-updated
-// state). beta == 0 stages skip the dissipation evaluation: diss*(1-0) + D*0
-// equals diss for finite fluxes.
-template <int FLUX, bool ORDER2, bool DISS, bool LAST>
-__global__ void __launch_bounds__(kWG) k_flux_update(DMesh m, DState s, Gas gas, double beta, double alpha,
-                                                     Free fs, int ntiles) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
-    double res2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (nf != 0) {
-        double wc[5], cv[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ds[5];
-        load_w(s.W, m.NA, c, wc);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) ds[v] = DISS ? s.diss[(size_t)v * m.NRp + c] * (1.0 - beta) : s.diss[(size_t)v * m.NRp + c];
-        for (int k = 0; k < nf; ++k) {
-            const Slot sl = slot(m, k, c);
-            double Sv[3], n[3], norm, rl[3], rr[3], wn[5], wl[5], wr[5], F[5];
-            load_w(s.W, m.NA, sl.nb, wn);
-            load3(m.S, m.NF, sl.f, Sv);
-            unit_normal(Sv[0], Sv[1], Sv[2], n, norm);
-            load3(m.rL, m.NF, sl.f, rl);
-            load3(m.rR, m.NF, sl.f, rr);
-            if (sl.right) {
-                recon_fused<ORDER2>(s, m.NA, sl.nb, wn, rl, wl);
-                recon_fused<ORDER2>(s, m.NA, c, wc, rr, wr);
-            } else {
-                recon_fused<ORDER2>(s, m.NA, c, wc, rl, wl);
-                recon_fused<ORDER2>(s, m.NA, sl.nb, wn, rr, wr);
-            }
-            if (FLUX == 0) central_flux(gas, wl, wr, n, norm, F);
-            else ausm_conv(gas, wl, wr, n, norm, F);
-            if (!sl.right) {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) cv[v] += F[v];
-            } else {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) cv[v] -= F[v];
-            }
-            if (DISS) {
-                double D[5];
-                if (FLUX == 0) roe_dissipation(gas, wl, wr, n, norm, D);
-                else ausm_diss(gas, wl, wr, n, norm, D);
-                if (!sl.right) {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) ds[v] += D[v] * beta;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) ds[v] -= D[v] * beta;
-                }
-            }
-        }
-        if (DISS) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) s.diss[(size_t)v * m.NRp + c] = ds[v];
-        }
-        if (LAST) {  // the last stage's convective residual feeds the monitors and the array getters
-#pragma unroll
-            for (int v = 0; v < 5; ++v) s.conv[(size_t)v * m.NRp + c] = cv[v];
-        }
-        double w0[5], wnew[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) w0[v] = s.W0[(size_t)v * m.NRp + c];
-        rk_update(gas.gm1, alpha * s.dt[c] / m.vol[c], w0, cv, ds, wnew);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + c] = wnew[v];
-        if (LAST) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                const double r = cv[v] - ds[v];
-                res2[v] = r * r;
-            }
-        }
-        for (int k = 0; k < nf; ++k) {  // ghosts of this cell: BC from the new state, or carry over
-            const Slot sl = slot(m, k, c);
-            if (sl.nb < m.NRp) continue;
-            const int kind = m.g_kind[sl.nb - m.NRp];
-            double g[5];
-            if (kind & GK_BC_MASK) apply_bc(m, kind, sl.f, gas.gamma, fs, wnew, g);
-            else load_w(s.W, m.NA, sl.nb, g);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + sl.nb] = g[v];
-        }
-    }
-    if (LAST) {  // per-tile partial sums of the squared residual (deterministic tree)
-        __shared__ double red[kWG / 64][5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            double x = res2[v];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
-        }
-        __syncthreads();
-        if (threadIdx.x < 5) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
-            s.part[(size_t)tile * 8 + threadIdx.x] = t;
-        }
     }
 }
 
@@ -669,8 +431,25 @@ __global__ void __launch_bounds__(kWG) k_block_monitors(const int* blk_tile0, co
 #pragma unroll
         for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
         out[(size_t)b * 16 + 5 + threadIdx.x] = t;
+    }
+    // residual norms: tile partials strided over the threads, then a fixed tree
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < blk_ntile[b]; q += kWG)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) r[v] += part[(size_t)(blk_tile0[b] + q) * 8 + v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        double x = r[v];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
         double rs = 0.0;
-        for (int q = 0; q < blk_ntile[b]; ++q) rs += part[(size_t)(blk_tile0[b] + q) * 8 + threadIdx.x];
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) rs += red[w][threadIdx.x];
         out[(size_t)b * 16 + threadIdx.x] = (1. / blk_nreal[b]) * sqrt(rs);
     }
 }
@@ -737,43 +516,6 @@ void launch_block_monitors(int nb, const int* tile0, const int* ntile, const int
                            const double* W, const double* part, double qcmac, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_block_monitors, dim3(nb), dim3(kWG), 0, st, tile0, ntile, nreal, wall_off, wall_ghost,
                        wall_face, fcenter, m.NF, m, W, part, qcmac, out);
-}
-
-void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, bool first, double k3, double omega,
-                         double gamma, double inv_gm1, double cfl, hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    const dim3 gr(xcd_grid(nt)), bl(kWG);
-#define IF_LAUNCH(G, V, F) hipLaunchKernelGGL((k_interp_fused<G, V, F>), gr, bl, 0, st, m, s, k3, omega, gamma, inv_gm1, cfl, nt)
-    if (grad == 0) {
-        if (venkat) { if (first) IF_LAUNCH(0, true, true); else IF_LAUNCH(0, true, false); }
-        else { if (first) IF_LAUNCH(0, false, true); else IF_LAUNCH(0, false, false); }
-    } else {
-        if (venkat) { if (first) IF_LAUNCH(1, true, true); else IF_LAUNCH(1, true, false); }
-        else { if (first) IF_LAUNCH(1, false, true); else IF_LAUNCH(1, false, false); }
-    }
-#undef IF_LAUNCH
-}
-
-void launch_prep_fused(const DMesh& m, const DState& s, double gamma, double inv_gm1, double cfl, hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    hipLaunchKernelGGL(k_prep_fused, dim3(xcd_grid(nt)), dim3(kWG), 0, st, m, s, gamma, inv_gm1, cfl, nt);
-}
-
-void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, const Gas& g,
-                        double beta, double alpha, const double fs[5], hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    const dim3 gr(xcd_grid(nt)), bl(kWG);
-    Free f;
-    for (int v = 0; v < 5; ++v) f.v[v] = fs[v];
-#define FU(FL, O2, DI, LA) hipLaunchKernelGGL((k_flux_update<FL, O2, DI, LA>), gr, bl, 0, st, m, s, g, beta, alpha, f, nt)
-#define FU_L(FL, O2, DI) { if (last) FU(FL, O2, DI, true); else FU(FL, O2, DI, false); }
-#define FU_D(FL, O2) { if (diss) FU_L(FL, O2, true) else FU_L(FL, O2, false) }
-#define FU_O(FL) { if (order2) FU_D(FL, true) else FU_D(FL, false) }
-    if (flux == 0) FU_O(0) else FU_O(1)
-#undef FU_O
-#undef FU_D
-#undef FU_L
-#undef FU
 }
 
 }  // namespace e3d
