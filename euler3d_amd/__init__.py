@@ -94,7 +94,11 @@ def lib():
             "e3d_init_freestream": (C.c_int, [C.c_void_p]),
             "e3d_get_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
             "e3d_set_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
-            "e3d_flux_diss": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+            "e3d_flux_diss": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+            "e3d_flux_conv": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "e3d_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "e3d_limiters": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+            "e3d_bc_marker": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "e3d_residual": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "e3d_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "e3d_exchange": (C.c_int, [C.c_void_p]),
@@ -106,9 +110,8 @@ def lib():
             "e3d_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_sync": (C.c_int, [C.c_void_p]),
         }
-        for name in ("e3d_save_w0", "e3d_spectral_radius", "e3d_timestep", "e3d_gradients",
-                     "e3d_limiters", "e3d_interpolation", "e3d_flux_conv", "e3d_smooth_residual",
-                     "e3d_bc", "e3d_runge_kutta"):
+        for name in ("e3d_save_w0", "e3d_spectral_radius", "e3d_timestep", "e3d_interpolation",
+                     "e3d_smooth_residual", "e3d_bc", "e3d_runge_kutta"):
             sig[name] = (C.c_int, [C.c_void_p, C.c_int])
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -58,6 +58,7 @@ struct e3d_ctx {
     // layout
     int nb = 0;
     std::vector<int> nreal, nall, roff, goff, foff, ghost_count;
+    std::vector<std::vector<int>> marker_off;  // per block: ghost offsets of its markers (relative to the block's first ghost)
     DMesh m{};
     DState s{};
     // device allocations
@@ -326,6 +327,7 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     c->goff.assign(nb, 0);
     c->foff.assign(nb, 0);
     c->ghost_count.assign(nb, 0);
+    c->marker_off.assign(nb, std::vector<int>(1, 0));
     long NRp = 0, NG = 0, NF = 0;
     int nslot = 1;
     for (int b = 0; b < nb; ++b) {
@@ -441,6 +443,8 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
             }
         }
         wall_off[b + 1] = (int)wall_ghost.size();
+        for (size_t j = 0; j < B.bnds.size(); ++j)
+            c->marker_off[b].push_back(c->marker_off[b].back() + (int)B.bnds[j].ghosts.size());
         // interfaces: destination ghosts numbered from GhostCount in topology order
         int k = 0;
         for (size_t i = 0; i < F.itf_origin_block.size(); ++i) {
@@ -594,40 +598,44 @@ int e3d_timestep(e3d_ctx* c, int block) {
     return launch_status(c);
 }
 
-int e3d_gradients(e3d_ctx* c, int block) {
+int e3d_gradients(e3d_ctx* c, int block, int kind) {
     int rc = check_ready(c, block);
     if (rc) return rc;
-    launch_gradient(block_range(c, block), c->m, c->s, c->params.gradient, c->stream);
+    if (kind != E3D_GRAD_GREEN_GAUSS && kind != E3D_GRAD_LEAST_SQUARES) return set_error(c, E3D_EINVAL, "unknown gradient");
+    launch_gradient(block_range(c, block), c->m, c->s, kind, c->stream);
     return launch_status(c);
 }
 
-int e3d_limiters(e3d_ctx* c, int block) {
+int e3d_limiters(e3d_ctx* c, int block, int kind, double param) {
     int rc = check_ready(c, block);
     if (rc) return rc;
-    const double k = c->params.k;
-    launch_limiter(block_range(c, block), c->m, c->s, c->params.limiter == E3D_LIM_VENKATAKRISHNAN, k * k * k,
-                   c->params.omega, c->stream);
+    if (kind != E3D_LIM_BARTH_JESPERSEN && kind != E3D_LIM_VENKATAKRISHNAN) return set_error(c, E3D_EINVAL, "unknown limiter");
+    const bool vk = kind == E3D_LIM_VENKATAKRISHNAN;
+    launch_limiter(block_range(c, block), c->m, c->s, vk, vk ? param * param * param : 0.0, vk ? 0.0 : param, c->stream);
     return launch_status(c);
 }
 
 int e3d_interpolation(e3d_ctx* c, int block) {
     int rc = check_ready(c, block);
-    if (rc || c->params.order != 2) return rc;
-    if ((rc = e3d_gradients(c, block))) return rc;
-    return e3d_limiters(c, block);
+    if (rc || c->params.order != 2) return rc;  // FirstOrder::computeInterpolation is a no-op
+    if ((rc = e3d_gradients(c, block, c->params.gradient))) return rc;
+    return e3d_limiters(c, block, c->params.limiter,
+                        c->params.limiter == E3D_LIM_VENKATAKRISHNAN ? c->params.k : c->params.omega);
 }
 
-int e3d_flux_conv(e3d_ctx* c, int block) {
+int e3d_flux_conv(e3d_ctx* c, int block, int scheme) {
     int rc = check_ready(c, block);
     if (rc) return rc;
-    launch_flux_conv(block_range(c, block), c->m, c->s, c->params.flux, c->gas, c->stream);
+    if (scheme != E3D_FLUX_ROE && scheme != E3D_FLUX_AUSM) return set_error(c, E3D_EINVAL, "unknown flux scheme");
+    launch_flux_conv(block_range(c, block), c->m, c->s, scheme, c->gas, c->stream);
     return launch_status(c);
 }
 
-int e3d_flux_diss(e3d_ctx* c, int block, double beta) {
+int e3d_flux_diss(e3d_ctx* c, int block, int scheme, double beta) {
     int rc = check_ready(c, block);
     if (rc) return rc;
-    launch_flux_diss(block_range(c, block), c->m, c->s, c->params.flux, c->gas, beta, c->stream);
+    if (scheme != E3D_FLUX_ROE && scheme != E3D_FLUX_AUSM) return set_error(c, E3D_EINVAL, "unknown flux scheme");
+    launch_flux_diss(block_range(c, block), c->m, c->s, scheme, c->gas, beta, c->stream);
     return launch_status(c);
 }
 
@@ -637,8 +645,8 @@ int e3d_residual(e3d_ctx* c, int block, int stage) {
     int rc = check_ready(c, block);
     if (rc) return rc;
     if (stage < 0 || stage >= c->params.stages) return set_error(c, E3D_EINVAL, "stage out of range");
-    if ((rc = e3d_interpolation(c, block)) || (rc = e3d_flux_conv(c, block)) ||
-        (rc = e3d_flux_diss(c, block, c->beta[stage])))
+    if ((rc = e3d_interpolation(c, block)) || (rc = e3d_flux_conv(c, block, c->params.flux)) ||
+        (rc = e3d_flux_diss(c, block, c->params.flux, c->beta[stage])))
         return rc;
     return e3d_smooth_residual(c, block);
 }
@@ -655,6 +663,19 @@ int e3d_bc(e3d_ctx* c, int block) {
     int rc = check_ready(c, block);
     if (rc) return rc;
     launch_bc(block_range(c, block), c->m, c->s, c->params.gamma, c->fs, c->stream);
+    return launch_status(c);
+}
+
+int e3d_bc_marker(e3d_ctx* c, int block, int marker) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    if (block < 0 || marker < 0 || marker >= (int)c->marker_off[block].size() - 1)
+        return set_error(c, E3D_EINVAL, "e3d_bc_marker: bad block or marker");
+    Range rg = block_range(c, block);
+    rg.c0 = rg.c1 = 0;
+    rg.g1 = rg.g0 + c->marker_off[block][marker + 1];
+    rg.g0 = rg.g0 + c->marker_off[block][marker];
+    launch_bc(rg, c->m, c->s, c->params.gamma, c->fs, c->stream);  // kernels skip ghosts without a BC
     return launch_status(c);
 }
 

@@ -146,16 +146,21 @@ int e3d_save_w0(e3d_ctx* ctx, int block);
 /* Timestep::computeSpectralRadius / computeTimestep (Solver/src/Timestep.cpp:26-148) */
 int e3d_spectral_radius(e3d_ctx* ctx, int block);
 int e3d_timestep(e3d_ctx* ctx, int block);
-/* Gradient::computeGradients (Solver/head/Gradient.h:15): GreenGauss.cpp:12-234 or LeastSquares.cpp:12-186 */
-int e3d_gradients(e3d_ctx* ctx, int block);
-/* Limiter::computeLimiters (Solver/head/Limiter.h:15): Venkatakrishnan.cpp:12-326 or BarthJespersen.cpp:14-333 */
-int e3d_limiters(e3d_ctx* ctx, int block);
+/* Gradient::computeGradients (Solver/head/Gradient.h:15): kind E3D_GRAD_GREEN_GAUSS
+ * (GreenGauss.cpp:12-234) or E3D_GRAD_LEAST_SQUARES (LeastSquares.cpp:12-186) */
+int e3d_gradients(e3d_ctx* ctx, int block, int kind);
+/* Limiter::computeLimiters (Solver/head/Limiter.h:15): kind E3D_LIM_VENKATAKRISHNAN with
+ * param = K (Venkatakrishnan.cpp:12-326) or E3D_LIM_BARTH_JESPERSEN with param = omega
+ * (BarthJespersen.cpp:14-333) */
+int e3d_limiters(e3d_ctx* ctx, int block, int kind, double param);
 /* Interpolation::computeInterpolation (Solver/head/Interpolation.h:15): FirstOrder no-op, SecondOrder gradient+limiter */
 int e3d_interpolation(e3d_ctx* ctx, int block);
-/* FluxScheme::computeFluxConv (Solver/head/FluxScheme.h:20): FluxScheme.cpp:16-211 (Roe) or AusmScheme.cpp:647-871 */
-int e3d_flux_conv(e3d_ctx* ctx, int block);
-/* FluxScheme::computeFluxDiss with current_beta_ (Solver/head/FluxScheme.h:19): RoeScheme.cpp:12-311 or AusmScheme.cpp:12-243 */
-int e3d_flux_diss(e3d_ctx* ctx, int block, double beta);
+/* FluxScheme::computeFluxConv (Solver/head/FluxScheme.h:20): scheme E3D_FLUX_ROE (the central
+ * flux of FluxScheme.cpp:16-211) or E3D_FLUX_AUSM (AusmScheme.cpp:647-871) */
+int e3d_flux_conv(e3d_ctx* ctx, int block, int scheme);
+/* FluxScheme::computeFluxDiss with current_beta_ = beta (Solver/head/FluxScheme.h:19):
+ * RoeScheme.cpp:12-311 or AusmScheme.cpp:12-243 */
+int e3d_flux_diss(e3d_ctx* ctx, int block, int scheme, double beta);
 /* ResidualSmoother::smoothResidual (Solver/head/ResidualSmoother.h:15): NoSmoother no-op */
 int e3d_smooth_residual(e3d_ctx* ctx, int block);
 /* ResidualCalculator::computeResidual (Solver/src/ResidualCalculator.cpp:24-41) for RK stage `stage` */
@@ -165,6 +170,9 @@ int e3d_update(e3d_ctx* ctx, int block, int stage);
 /* Updater::updateBoundary -> BoundaryCellIds::updateBoundary for every physical marker
  * (Solver/src/Updater.cpp:80-88; WallCellIds.cpp:9-95, SymmetryCellIds.cpp:9-96, FarfieldCellIds.cpp:10-114) */
 int e3d_bc(e3d_ctx* ctx, int block);
+/* BoundaryCellIds::updateBoundary (DataStructure/head/BoundaryCellIds.h:13) of one marker of a
+ * block (marker index in file order); a CONNEXION marker is a no-op as in ConnexionBlockCellIds. */
+int e3d_bc_marker(e3d_ctx* ctx, int block, int marker);
 /* RungeKutta::computeRungeKutta (Solver/src/RungeKutta.cpp:80-94) */
 int e3d_runge_kutta(e3d_ctx* ctx, int block);
 /* BlockCommunicator::updateBoundaries (MPI/src/BlockCommunicator.cpp:212-268): interface ghosts

@@ -137,3 +137,39 @@ def oracle_state(B, what):
 def values_equal(a, b):
     """Elementwise equality treating +0 == -0 and NaN == NaN."""
     return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+FLUX_NAMES = {0: "roe", 1: "ausm"}
+GRAD_NAMES = {0: "greengauss", 1: "leastsquares"}
+LIM_NAMES = {0: "barthjespersen", 1: "venkatakrishnan"}
+
+
+def write_input(path, topology, meta, max_iter=None, criterion=-15.0):
+    """Writes a solver input file in the reference's positional 32-line layout
+    (input_onera120:1-32) for the parameters of a golden case."""
+    it = meta["iters"] if max_iter is None else max_iter
+    lines = ["MESH", "topologyfilename", topology, "INPUT", "cfl gamma angleattackdeg",
+             f"1.5 1.4 {meta['aoa']!r}", "rkstage mach cmac", f"{meta['stages']} {meta['mach']!r} 1.0",
+             "nbitermax convcriterion", f"{it} {criterion!r}", "residual smoothing (0-no 1-yes)", "0",
+             "fluxscheme schemeorder", f"{FLUX_NAMES[meta['flux']]} {meta['order']}", "gradient limiter",
+             f"{GRAD_NAMES[meta['grad']]} {LIM_NAMES[meta['lim']]}", "omega k", f"{meta['omega']!r} {meta['k']!r}",
+             "EXECUTABLE (0-no 1-yes)", "build execute", "1 1", "OUTPUT (0-no 1-yes)", "clalpha cdalpha cmalpha",
+             "1 1 1", "coefficientsconv  residualconv", "1 1", "cpxc machcontour cpcontour machisosurface",
+             "1 1 1 0", "slicecp", "1", "axis xcoord ycoord zcoord", "x 1.00 1.00 1.00"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_case_mesh(g, directory):
+    """Writes the mesh of a golden case (SU2 blocks + topology) and returns the topology path."""
+    if g.meta["mesh"] == "naca":
+        su2 = os.path.join(directory, "naca.su2")
+        write_su2_from_fixture(g, su2)
+        top = os.path.join(directory, "naca.top")
+        with open(top, "w") as f:
+            f.write(f"NBLOCK= 1\n{su2}\nBlock= 0\nGhostCount= 0\nNconnexion= 0\n")
+        return top
+    m = mesh_for(g)
+    top = m.write(directory, "case")
+    m.close()
+    return top
