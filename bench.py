@@ -10,11 +10,14 @@ whole mesh, launched through e3d_iterate; the mesh and state are resident in
 HBM before the timed region starts.
 
 metric: Mcell-RK-stage/s = real cells x RK stages x steps / wall time.
-roofline: the dominant kernel (flux_update: residual + RK update + BCs on a
-stage with dissipation), algorithmic bytes per launch / average launch time
-from HIP events recorded on the context's stream during the timed region.
+roofline: the kernel with the largest share of the timed region (interp:
+gradient + limiter + face states, or flux_update: residual + RK update + BCs
+on a stage with dissipation), algorithmic bytes per launch / average launch
+time from HIP events recorded on the context's stream during the timed region;
+"rooflines" carries the same figures for both kernels.
 cpu_baseline: the oracle (plain-C restatement, one core) on a bounded sample of
-the same workload: one of the 8 blocks (589,824 cells) for one RK iteration.
+the same workload: one of the 8 blocks (589,824 cells) for 4 RK iterations
+(about 10 s).
 """
 from __future__ import annotations
 
@@ -28,23 +31,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MODEL_BYTES_PER_CELL_STAGE = 1110.6  # compulsory-traffic model of this path at this workload
+# Compulsory bytes per cell and RK stage of the whole second-order Roe/GG/VK
+# stage at this workload (K_A + K_B + K_C + per-iteration work / 5, below).
+MODEL_BYTES_PER_CELL_STAGE = 1110.6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_kernels_bump192x64x64_traffic.json")
 
 
 def algorithmic_bytes(blocks):
-    """Algorithmic bytes of one flux_update launch (a stage with dissipation).
+    """Algorithmic bytes per launch of the two stage kernels.
 
     The compulsory-traffic model of this path: every array element counted once
     per kernel that must read or write it (W 40 B/cell, grad 120, lim 40, W0 40,
     conv+diss 80, dt 8, V 8; per face S 24 + rL, rR 48 + left/right 8 = 80 B).
+    interp is the gradient + limiter kernel K_A = 208(N+G) + 80F + 16N;
     flux_update does the residual kernel K_B = 200(N+G) + 80F + 120N and the
     update + BC kernel K_C = 176N + 112G of that model in one launch."""
     N = sum(b["nreal"] for b in blocks)
     G = sum(b["nall"] - b["nreal"] for b in blocks)
     F = sum(b["nface"] for b in blocks)
+    ka = 208 * (N + G) + 80 * F + 16 * N
     kb = 200 * (N + G) + 80 * F + 120 * N
     kc = 176 * N + 112 * G
-    return kb + kc, N, G, F
+    return {"interp": ka, "flux_update": kb + kc}, N, G, F
 
 
 def main():
@@ -63,8 +71,8 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the multi-GPU "
-              "layout is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (blocks would shard "
+              "across GPUs by contiguous ranges, but that path is not built)", file=sys.stderr)
         sys.exit(2)
 
     import euler3d_amd as E
@@ -73,7 +81,7 @@ def main():
     mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, 4, tuple(a.parts))
     blocks = [mesh.block(b) for b in range(mesh.nblocks)]
     t_mesh = time.perf_counter() - t_mesh
-    bytes_fu, N, G, F = algorithmic_bytes(blocks)
+    alg, N, G, F = algorithmic_bytes(blocks)
     params = E.make_params(0.675, 0.0, flux=E.FLUX_ROE, order=2, gradient=E.GRAD_GREEN_GAUSS,
                            limiter=E.LIM_VENKATAKRISHNAN, k=5.0, stages=5, max_iter=10 ** 9, criterion=-1e9)
     ctx = E.Context(0)
@@ -97,14 +105,19 @@ def main():
     for name in ("interp0", "interp", "flux_update", "flux_update_nodiss", "exchange", "monitors"):
         ms, n = ctx.kernel_time(name)
         kt[name] = {"avg_ms": ms, "launches": n}
-    fu = kt["flux_update"]
-    bytes_per_launch = bytes_fu
-    achieved = bytes_per_launch / (fu["avg_ms"] * 1e-3) / 1e9 if fu["avg_ms"] > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "flux_update_c2_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    measured = {}
+    if os.path.exists(TRAFFIC_FILE) and (a.nx, a.ny, a.nz, tuple(a.parts)) == (192, 64, 64, (2, 2, 2)):
+        with open(TRAFFIC_FILE) as f:
+            measured = json.load(f)["hbm_bytes_per_launch"]
+    rooflines = {}
+    for name in ("interp", "flux_update"):
+        t = kt[name]
+        ach = alg[name] / (t["avg_ms"] * 1e-3) / 1e9 if t["avg_ms"] > 0 else 0.0
+        rooflines[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": measured.get(name),
+                           "algorithmic_bytes_per_launch": alg[name], "avg_ms": t["avg_ms"],
+                           "share_of_step": t["avg_ms"] * t["launches"] / (wall * 1e3)}
+    dominant = max(rooflines, key=lambda k: rooflines[k]["share_of_step"])
     rms_ro = hist[-1]["rms"][0] if hist else float("nan")
     print(f"[bench] {a.steps} steps in {wall:.3f}s; rms_ro {rms_ro:.6e}; kernels "
           + ", ".join(f"{k} {v['avg_ms']:.3f}ms x{v['launches']}" for k, v in kt.items()), file=sys.stderr, flush=True)
@@ -148,9 +161,8 @@ def main():
                                "Venkatakrishnan K=5, M=0.675, 5-stage RK, CFL 1.5",
                    "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages},
         "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / MODEL_BYTES_PER_CELL_STAGE / 1e6),
-        "roofline": {"bound": "hbm", "kernel": "flux_update", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_ms": fu["avg_ms"]},
+        "roofline": rooflines[dominant],
+        "rooflines": rooflines,
         "kernels": kt,
         "cpu_baseline": cpu,
     }
