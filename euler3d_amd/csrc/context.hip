@@ -29,21 +29,19 @@ void launch_exchange(int, const int*, const int*, int, double*, double*, double*
 void launch_rms_partials(const DMesh&, const DState&, hipStream_t);
 void launch_block_monitors(int, const int*, const int*, const int*, const int*, const int*, const int*,
                            const double*, const DMesh&, const double*, const double*, double, double*, hipStream_t);
-void launch_interp_fused(const DMesh&, const DState&, int, int, bool, double, double, double, double, double,
-                         hipStream_t);
-void launch_prep_fused(const DMesh&, const DState&, double, double, double, hipStream_t);
-void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, const Gas&, double, double,
-                        const double*, hipStream_t);
+void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
+void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, bool, const Gas&, double, double,
+                        const double*, double, double, hipStream_t);
 }  // namespace e3d
 
 using namespace e3d;
 
 namespace {
-enum KernelId { KI_INTERP0, KI_INTERP, KI_PREP, KI_FLUX_UPDATE, KI_FLUX_UPDATE_ND, KI_EXCHANGE, KI_MONITORS, KI_COUNT };
-// interp0 / interp: stage-0 (with W0 and time step) and later stages; flux_update /
-// flux_update_nodiss: stages with beta != 0 and beta == 0.
-const char* kKernelNames[KI_COUNT] = {"interp0", "interp", "prep", "flux_update", "flux_update_nodiss",
-                                      "exchange", "monitors"};
+enum KernelId { KI_INTERP, KI_FLUX_UPDATE0, KI_FLUX_UPDATE, KI_FLUX_UPDATE_ND, KI_EXCHANGE, KI_MONITORS, KI_COUNT };
+// flux_update0: stage 0 (with W0, spectral radius and time step); flux_update /
+// flux_update_nodiss: later stages with beta != 0 and beta == 0.
+const char* kKernelNames[KI_COUNT] = {"interp", "flux_update0", "flux_update", "flux_update_nodiss", "exchange",
+                                      "monitors"};
 }  // namespace
 
 struct e3d_ctx {
@@ -399,7 +397,8 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
                 const double* rv = right ? &B.rR[3 * f] : &B.rL[3 * f];
                 const double sv[3] = {sx, sy, sz};
                 for (int d = 0; d < 3; ++d) {
-                    sS[((size_t)d * nslot + k) * NR + gc] = sv[d];
+                    // oriented away from this cell, as the gradient uses it (GreenGauss.cpp:104-118)
+                    sS[((size_t)d * nslot + k) * NR + gc] = right ? sv[d] * -1.0 : sv[d];
                     sR[((size_t)d * nslot + k) * NR + gc] = rv[d];
                     sN[((size_t)d * nslot + k) * NR + gc] = sv[d] / norm;
                 }
@@ -736,18 +735,14 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
         for (int st = 0; st < p.stages; ++st) {
             size_t ev = 0;
             if (order2) {
-                ev_begin(c, st == 0 ? KI_INTERP0 : KI_INTERP, &ev);
-                launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, st == 0, k * k * k,
-                                    p.omega, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
-                ev_end(c, ev);
-            } else if (st == 0) {
-                ev_begin(c, KI_PREP, &ev);
-                launch_prep_fused(c->m, c->s, p.gamma, 1. / (p.gamma - 1.0), p.cfl, c->stream);
+                ev_begin(c, KI_INTERP, &ev);
+                launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, k * k * k, p.omega,
+                                    c->stream);
                 ev_end(c, ev);
             }
-            ev_begin(c, c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
-            launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, c->gas, c->beta[st],
-                               c->alpha[st], c->fs, c->stream);
+            ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
+            launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, st == 0, c->gas,
+                               c->beta[st], c->alpha[st], c->fs, p.cfl, 1. / (p.gamma - 1.0), c->stream);
             ev_end(c, ev);
             if ((rc = launch_status(c))) return rc;
             std::swap(c->s.W, c->s.Wn);

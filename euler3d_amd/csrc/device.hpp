@@ -37,7 +37,7 @@ struct DMesh {
     // Per-slot copies of face geometry for the fused path, slot-major so a
     // workgroup streams them coalesced: component q of slot k of cell c at
     // (q*nslot + k)*NRp + c.
-    const double* sS;     // [3][nslot][NRp] area vector S of the face (its own orientation, left -> right)
+    const double* sS;     // [3][nslot][NRp] area vector S of the face, negated when this cell is its right cell
     const double* sR;     // [3][nslot][NRp] r vector of the face on this cell's side
     const double* sN;     // [3][nslot][NRp] unit normal S/|S|
     const double* sA;     // [nslot][NRp] |S|

@@ -4,11 +4,12 @@
 //              cell, reflected gradient and limiter of the ghosts hanging off
 //              it, and the reconstructed state W + (lim*grad).r of the cell on
 //              each of its faces, written as one 5-double record per
-//              (cell, face). On stage 0 also saveW0 and the local time step.
+//              (cell, face).
 //   k_flux   : per cell and face, the flux from the cell's own face-state
 //              record and the other cell's record (one 40-byte gather),
 //              accumulated in the reference's face order; then the RK update
-//              and the ghost BCs, into the second state buffer.
+//              and the ghost BCs, into the second state buffer. On stage 0
+//              also saveW0, the spectral radius and the local time step.
 //
 // Face geometry is stored per (cell, face) slot so it streams coalesced; the
 // only gathers are the neighbour states in k_interp and one face-state record
@@ -50,59 +51,53 @@ __device__ __forceinline__ void face_state(const double w[5], const double lim[5
         for (int v = 0; v < 5; ++v) out[v] += lim[v] * g[v * 3 + d] * r[d];
 }
 
+// Limiter, reconstructed state and record of the ghost hanging off face slot k
+// of real cell c, one variable at a time: reflected (or zero) gradient
+// (GreenGauss.cpp:175-234), the ghost's limiter over its single neighbour c
+// (Venkatakrishnan.cpp:87-243 run over a ghost), and W + (lim*grad).r with the
+// ghost's reflected r vector.
 template <bool VENKAT>
-struct Lim {
-    double mx[5], mn[5], wc[5], lim[5];
-    __device__ __forceinline__ void start(const double w[5]) {
+__device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s, int k, int c, int nb,
+                                                 const double g[15], const double wc[5], double eps2, double omega) {
+    const int NRp = m.NRp, ns = m.nslot;
+    const int gi = nb - NRp;
+    const bool reflect = m.g_kind[gi] & GK_REFLECT;
+    double n[3], rg[3];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            wc[v] = w[v];
-            mx[v] = w[v];
-            mn[v] = w[v];
-            lim[v] = __builtin_inf();
+    for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
+    load3(m.g_r, m.NG, gi, rg);
+    double* rec = s.Wf + ghost_record(ns, NRp, gi) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        double gg[3];
+        if (reflect) {
+            double dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) dot += g[v * 3 + d] * n[d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) gg[d] = g[v * 3 + d] - 2.0 * dot * n[d];
+        } else {
+            gg[0] = gg[1] = gg[2] = 0.0;
         }
-    }
-    __device__ __forceinline__ void neighbour(const double wn[5]) {
+        const double wg = s.W[(size_t)v * m.NA + nb];
+        double mx = wg, mn = wg;
+        if (wc[v] > mx) mx = wc[v];
+        if (wc[v] < mn) mn = wc[v];
+        double d2 = 0.0;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            if (wn[v] > mx[v]) mx[v] = wn[v];
-            if (wn[v] < mn[v]) mn[v] = wn[v];
-        }
-    }
-    __device__ __forceinline__ void face(const double g[15], const double r[3], double eps2, double omega) {
+        for (int d = 0; d < 3; ++d) d2 += gg[d] * rg[d];
+        const double phi = face_limiter<VENKAT>(d2, mx - wg, mn - wg, eps2, omega);
+        double lim = __builtin_inf();
+        if (phi < lim) lim = phi;
+        double out = wg;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            double d2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) d2 += g[v * 3 + d] * r[d];
-            const double phi = face_limiter<VENKAT>(d2, mx[v] - wc[v], mn[v] - wc[v], eps2, omega);
-            if (phi < lim[v]) lim[v] = phi;
-        }
+        for (int d = 0; d < 3; ++d) out += lim * gg[d] * rg[d];
+        rec[v] = out;
     }
-};
-
-__device__ __forceinline__ void save_w0_dt(const DMesh& m, const DState& s, int c, int nf, const double wc[5],
-                                           double vol, double gamma, double inv_gm1, double cfl) {
-    const size_t N = m.NRp;
-    s.W0[c] = wc[0];
-    s.W0[N + c] = wc[0] * wc[1];
-    s.W0[2 * N + c] = wc[0] * wc[2];
-    s.W0[3 * N + c] = wc[0] * wc[3];
-    s.W0[4 * N + c] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
-    // Timestep::computeSpectralRadius (Timestep.cpp:67-119) and computeTimestep (:139-146)
-    const double c_center = sqrt(gamma * wc[4] / wc[0]);
-    double sr = 0.0;
-    for (int k = 0; k < nf; ++k) {
-        const double nx = slotv(m.sN, 0, m.nslot, m.NRp, k, c), ny = slotv(m.sN, 1, m.nslot, m.NRp, k, c),
-                     nz = slotv(m.sN, 2, m.nslot, m.NRp, k, c);
-        sr += (fabs((wc[1]) * nx + (wc[2]) * ny + (wc[3]) * nz) + (c_center)) * m.sA[(size_t)k * m.NRp + c];
-    }
-    s.dt[c] = cfl * vol / sr;
 }
 
-template <int GRAD, bool VENKAT, bool FIRST, int MAXS>
-__global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, double omega, double gamma,
-                                                double inv_gm1, double cfl, int ntiles) {
+template <int GRAD, bool VENKAT, int MAXS>
+__global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const int c = tile * kWG + threadIdx.x;
@@ -111,27 +106,28 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     const int NRp = m.NRp, ns = m.nslot;
     double wc[5];
     load_w(s.W, m.NA, c, wc);
+    const double vol = m.vol[c];
     // pass 1: gradient and the neighbour extrema, faces in the reference's order
-    double g[15];
+    double g[15], mx[5], mn[5];
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
-    Lim<VENKAT> L;
-    L.start(wc);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) mx[v] = mn[v] = wc[v];
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
             double wn[5];
             load_w(s.W, m.NA, m.nbr[(size_t)k * NRp + c], wn);
-            L.neighbour(wn);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (wn[v] > mx[v]) mx[v] = wn[v];
+                if (wn[v] < mn[v]) mn[v] = wn[v];
+            }
             if (GRAD == 0) {
+                // S oriented away from this cell (negated when the cell is the face's right cell)
                 double n[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) n[d] = slotv(m.sS, d, ns, NRp, k, c);
-                if (m.mate[(size_t)k * NRp + c] < 0) {  // this cell is the face's right cell
-                    n[0] *= -1.0;
-                    n[1] *= -1.0;
-                    n[2] *= -1.0;
-                }
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -146,20 +142,33 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
                     for (int d = 0; d < 3; ++d) g[v * 3 + d] += wt[d] * (wn[v] - wc[v]);
             }
         }
-    const double vol = m.vol[c];
     if (GRAD == 0) {
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] /= vol;
     }
     // pass 2: limiter over the faces
     const double eps2 = k3 * (vol);
+    double lim[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        mx[v] -= wc[v];
+        mn[v] -= wc[v];
+        lim[v] = __builtin_inf();
+    }
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
             double r[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) r[d] = slotv(m.sR, d, ns, NRp, k, c);
-            L.face(g, r, eps2, omega);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) d2 += g[v * 3 + d] * r[d];
+                const double phi = face_limiter<VENKAT>(d2, mx[v], mn[v], eps2, omega);
+                if (phi < lim[v]) lim[v] = phi;
+            }
         }
     // pass 3: face states of the cell; ghosts hanging off it
 #pragma unroll
@@ -168,59 +177,20 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
             double r[3], out[5];
 #pragma unroll
             for (int d = 0; d < 3; ++d) r[d] = slotv(m.sR, d, ns, NRp, k, c);
-            face_state(wc, L.lim, g, r, out);
+            face_state(wc, lim, g, r, out);
             store_rec(s.Wf, (size_t)k * NRp + c, out);
             const int nb = m.nbr[(size_t)k * NRp + c];
-            if (nb < NRp) continue;
-            // ghost: reflected (or zero) gradient, its limiter, its face state
-            const int gi = nb - NRp;
-            const int kind = m.g_kind[gi];
-            double gg[15];
-            if (kind & GK_REFLECT) {
-                double n[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    double dot = 0.0;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) dot += g[v * 3 + d] * n[d];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) gg[v * 3 + d] = g[v * 3 + d] - 2.0 * dot * n[d];
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 15; ++i) gg[i] = 0.0;
-            }
-            double rg[3], wg[5];
-            load3(m.g_r, m.NG, gi, rg);
-            load_w(s.W, m.NA, nb, wg);
-            Lim<VENKAT> LG;
-            LG.start(wg);
-            LG.neighbour(wc);
-            LG.face(gg, rg, eps2, omega);  // a ghost's volume is its interior cell's
-            face_state(wg, LG.lim, gg, rg, out);
-            store_rec(s.Wf, ghost_record(ns, NRp, gi), out);
+            if (nb >= NRp) ghost_face_state<VENKAT>(m, s, k, c, nb, g, wc, eps2, omega);  // a ghost's volume is its interior cell's
         }
-    if (FIRST) save_w0_dt(m, s, c, nf, wc, vol, gamma, inv_gm1, cfl);
 }
 
-// Stage 0 of a first-order run: saveW0 + spectral radius + time step.
-__global__ void __launch_bounds__(kWG) k_prep(DMesh m, DState s, double gamma, double inv_gm1, double cfl,
-                                              int ntiles) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
-    if (nf == 0) return;
-    double wc[5];
-    load_w(s.W, m.NA, c, wc);
-    save_w0_dt(m, s, c, nf, wc, m.vol[c], gamma, inv_gm1, cfl);
-}
-
-template <int FLUX, bool ORDER2, bool DISS, bool LAST, int MAXS>
+// FIRST (stage 0) also does what the reference runs once per iteration before
+// the stages: saveW0 (Solver.cpp:161-210) and the spectral radius and local
+// time step (Timestep.cpp:67-146), all from the cell's own state at the start
+// of the iteration, with the spectral radius summed in the same face loop.
+template <int FLUX, bool ORDER2, bool DISS, bool LAST, bool FIRST, int MAXS>
 __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double beta, double alpha, Free fs,
-                                              int ntiles) {
+                                              double cfl, double inv_gm1, int ntiles) {
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const int c = tile * kWG + threadIdx.x;
@@ -230,7 +200,9 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
     double res2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (nf != 0) {
         double cv[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ds[5], wc[5];
-        if (!ORDER2) load_w(s.W, m.NA, c, wc);
+        if (!ORDER2 || FIRST) load_w(s.W, m.NA, c, wc);
+        double c_center = 0.0, sr = 0.0;
+        if (FIRST) c_center = sqrt(gas.gamma * wc[4] / wc[0]);
 #pragma unroll
         for (int v = 0; v < 5; ++v)
             ds[v] = DISS ? s.diss[(size_t)v * NRp + c] * (1.0 - beta) : s.diss[(size_t)v * NRp + c];
@@ -244,6 +216,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
 #pragma unroll
                 for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);
                 const double norm = m.sA[(size_t)k * NRp + c];
+                if (FIRST) sr += (fabs((wc[1]) * n[0] + (wc[2]) * n[1] + (wc[3]) * n[2]) + (c_center)) * norm;
                 if (ORDER2) {
                     load_rec(s.Wf, (size_t)k * NRp + c, own);
                     load_rec(s.Wf, rec, oth);
@@ -285,10 +258,24 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
 #pragma unroll
             for (int v = 0; v < 5; ++v) s.conv[(size_t)v * NRp + c] = cv[v];
         }
-        double w0[5], wnew[5];
+        double w0[5], wnew[5], dt;
+        const double vol = m.vol[c];
+        if (FIRST) {
+            w0[0] = wc[0];
+            w0[1] = wc[0] * wc[1];
+            w0[2] = wc[0] * wc[2];
+            w0[3] = wc[0] * wc[3];
+            w0[4] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
+            dt = cfl * vol / sr;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) w0[v] = s.W0[(size_t)v * NRp + c];
-        rk_update(gas.gm1, alpha * s.dt[c] / m.vol[c], w0, cv, ds, wnew);
+            for (int v = 0; v < 5; ++v) s.W0[(size_t)v * NRp + c] = w0[v];
+            s.dt[c] = dt;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) w0[v] = s.W0[(size_t)v * NRp + c];
+            dt = s.dt[c];
+        }
+        rk_update(gas.gm1, alpha * dt / vol, w0, cv, ds, wnew);
 #pragma unroll
         for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + c] = wnew[v];
         if (LAST) {
@@ -342,36 +329,32 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
 
 // ------------------------------------------------------------ host launchers
 
-void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, bool first, double k3, double omega,
-                         double gamma, double inv_gm1, double cfl, hipStream_t st) {
+void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, double k3, double omega,
+                         hipStream_t st) {
     const int nt = m.NRp / kWG;
     const dim3 gr(xcd_grid(nt)), bl(kWG);
-#define IL(G, V, F, S) hipLaunchKernelGGL((k_interp<G, V, F, S>), gr, bl, 0, st, m, s, k3, omega, gamma, inv_gm1, cfl, nt)
-#define IL_S(G, V, F) { if (m.nslot <= 4) IL(G, V, F, 4); else IL(G, V, F, 6); }
-#define IL_F(G, V) { if (first) IL_S(G, V, true) else IL_S(G, V, false) }
-#define IL_V(G) { if (venkat) IL_F(G, true) else IL_F(G, false) }
+#define IL(G, V, S) hipLaunchKernelGGL((k_interp<G, V, S>), gr, bl, 0, st, m, s, k3, omega, nt)
+#define IL_S(G, V) { if (m.nslot <= 4) IL(G, V, 4); else IL(G, V, 6); }
+#define IL_V(G) { if (venkat) IL_S(G, true) else IL_S(G, false) }
     if (grad == 0) IL_V(0) else IL_V(1)
 #undef IL_V
-#undef IL_F
 #undef IL_S
 #undef IL
 }
 
-void launch_prep_fused(const DMesh& m, const DState& s, double gamma, double inv_gm1, double cfl, hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    hipLaunchKernelGGL(k_prep, dim3(xcd_grid(nt)), dim3(kWG), 0, st, m, s, gamma, inv_gm1, cfl, nt);
-}
-
-void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, const Gas& g,
-                        double beta, double alpha, const double fs[5], hipStream_t st) {
+void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, bool first,
+                        const Gas& g, double beta, double alpha, const double fs[5], double cfl, double inv_gm1,
+                        hipStream_t st) {
     const int nt = m.NRp / kWG;
     const dim3 gr(xcd_grid(nt)), bl(kWG);
     Free f;
     for (int v = 0; v < 5; ++v) f.v[v] = fs[v];
-#define FL(FX, O2, DI, LA, S) hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, S>), gr, bl, 0, st, m, s, g, beta, alpha, f, nt)
-#define FL_S(FX, O2, DI, LA) { if (m.nslot <= 4) FL(FX, O2, DI, LA, 4); else FL(FX, O2, DI, LA, 6); }
-#define FL_L(FX, O2, DI) { if (last) FL_S(FX, O2, DI, true) else FL_S(FX, O2, DI, false) }
-#define FL_D(FX, O2) { if (diss) FL_L(FX, O2, true) else FL_L(FX, O2, false) }
+    // Stage 0 always has dissipation (beta_0 = 1 in every RK table), so FIRST implies DISS.
+#define FL(FX, O2, DI, LA, FI, S) \
+    hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, FI, S>), gr, bl, 0, st, m, s, g, beta, alpha, f, cfl, inv_gm1, nt)
+#define FL_S(FX, O2, DI, LA, FI) { if (m.nslot <= 4) FL(FX, O2, DI, LA, FI, 4); else FL(FX, O2, DI, LA, FI, 6); }
+#define FL_L(FX, O2, DI, FI) { if (last) FL_S(FX, O2, DI, true, FI) else FL_S(FX, O2, DI, false, FI) }
+#define FL_D(FX, O2) { if (first) FL_L(FX, O2, true, true) else if (diss) FL_L(FX, O2, true, false) else FL_L(FX, O2, false, false) }
 #define FL_O(FX) { if (order2) FL_D(FX, true) else FL_D(FX, false) }
     if (flux == 0) FL_O(0) else FL_O(1)
 #undef FL_O
