@@ -209,8 +209,8 @@ def read_input(path):
 
 def make_params(mach, aoa_deg=0.0, flux=FLUX_ROE, order=1, gradient=GRAD_GREEN_GAUSS,
                 limiter=LIM_VENKATAKRISHNAN, omega=1e-16, k=5.0, gamma=1.4, cfl=1.5, stages=5,
-                cmac=1.0, max_iter=200, criterion=-8.0):
-    return Params(gamma, cfl, aoa_deg, mach, cmac, stages, max_iter, criterion, 0, flux, order,
+                cmac=1.0, max_iter=200, criterion=-8.0, smoothing=0):
+    return Params(gamma, cfl, aoa_deg, mach, cmac, stages, max_iter, criterion, smoothing, flux, order,
                   gradient, limiter, omega, k)
 
 

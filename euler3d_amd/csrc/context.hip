@@ -27,6 +27,7 @@ void launch_update(const Range&, const DMesh&, const DState&, double, double, hi
 void launch_bc(const Range&, const DMesh&, const DState&, double, const double*, hipStream_t);
 void launch_exchange(int, const int*, const int*, int, double*, double*, double*, int, hipStream_t);
 void launch_rms_partials(const DMesh&, const DState&, hipStream_t);
+void launch_smooth(const Range&, const DMesh&, const DState&, hipStream_t);
 void launch_block_monitors(int, const int*, const int*, const int*, const int*, const int*, const int*,
                            const double*, const DMesh&, const double*, const double*, double, double*, hipStream_t);
 void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
@@ -284,7 +285,7 @@ int e3d_set_params(e3d_ctx* c, const e3d_params* p) {
         return set_error(c, E3D_EINVAL, "unknown gradient");
     if (p->limiter != E3D_LIM_BARTH_JESPERSEN && p->limiter != E3D_LIM_VENKATAKRISHNAN)
         return set_error(c, E3D_EINVAL, "unknown limiter");
-    if (p->smoothing != 0) return set_error(c, E3D_EINVAL, "residual smoothing is not supported (NoSmoother only)");
+    if (p->smoothing != 0 && p->smoothing != 1) return set_error(c, E3D_EINVAL, "smoothing must be 0 or 1");
     if (!(p->gamma > 1.0)) return set_error(c, E3D_EINVAL, "gamma must exceed 1");
     c->params = *p;
     // RungeKutta constructor tables (Solver/src/RungeKutta.cpp:23-68)
@@ -478,6 +479,7 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
     m.sS = d_sS; m.sR = d_sR; m.sN = d_sN; m.sA = d_sA; m.mate = d_mate; m.g_r = d_gr;
     DState& s = c->s;
+    s.jac = nullptr;
     if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
         (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
         (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.Wf, 5 * (NS + NG))) || (rc = dalloc(c, &s.dt, NR)) ||
@@ -638,7 +640,13 @@ int e3d_flux_diss(e3d_ctx* c, int block, int scheme, double beta) {
     return launch_status(c);
 }
 
-int e3d_smooth_residual(e3d_ctx* c, int block) { return check_ready(c, block); }
+int e3d_smooth_residual(e3d_ctx* c, int block) {
+    int rc = check_ready(c, block);
+    if (rc) return rc;
+    if (!c->s.jac && (rc = dalloc(c, &c->s.jac, 10 * (size_t)c->m.NRp))) return rc;
+    launch_smooth(block_range(c, block), c->m, c->s, c->stream);
+    return launch_status(c);
+}
 
 int e3d_residual(e3d_ctx* c, int block, int stage) {
     int rc = check_ready(c, block);
@@ -647,7 +655,7 @@ int e3d_residual(e3d_ctx* c, int block, int stage) {
     if ((rc = e3d_interpolation(c, block)) || (rc = e3d_flux_conv(c, block, c->params.flux)) ||
         (rc = e3d_flux_diss(c, block, c->params.flux, c->beta[stage])))
         return rc;
-    return e3d_smooth_residual(c, block);
+    return c->params.smoothing ? e3d_smooth_residual(c, block) : E3D_OK;  // NoSmoother otherwise
 }
 
 int e3d_update(e3d_ctx* c, int block, int stage) {
@@ -717,11 +725,40 @@ int e3d_request_stop(e3d_ctx* c) {
 
 // ------------------------------------------------------------- fused path
 
+// Solver::solve with one granular operator call per reference method, block by
+// block (Solver/src/Solver.cpp:76-98). Used by e3d_iterate when the residual
+// smoother is on: its Jacobi sweep reads the neighbours' residuals, which the
+// fused stage kernels do not hold.
+static int iterate_granular(e3d_ctx* c, int n, e3d_monitor* history) {
+    int rc;
+    if (!c->started) {  // Solver.cpp:48
+        if ((rc = exchange(c, 1))) return rc;
+        c->started = true;
+    }
+    int done = 0;
+    for (int it = 0; it < n; ++it) {
+        if (c->stop_request.exchange(0)) break;
+        for (int b = 0; b < c->nb; ++b)
+            if ((rc = e3d_save_w0(c, b)) || (rc = e3d_spectral_radius(c, b)) || (rc = e3d_timestep(c, b)) ||
+                (rc = e3d_runge_kutta(c, b)))
+                return rc;
+        if ((rc = exchange(c, 1))) return rc;  // Solver.cpp:94
+        e3d_monitor mon;
+        rc = e3d_monitors(c, &mon);
+        if (history) history[done] = mon;
+        ++done;
+        if (rc) return rc;
+        if (mon.stop) break;
+    }
+    return done;
+}
+
 int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
     int rc = check_ready(c, -1);
     if (rc) return rc;
     if (n < 0) return set_error(c, E3D_EINVAL, "negative iteration count");
     HIP_TRY(c, hipSetDevice(c->device));
+    if (c->params.smoothing) return iterate_granular(c, n, history);
     const e3d_params& p = c->params;
     const bool order2 = p.order == 2;
     const double k = p.k;

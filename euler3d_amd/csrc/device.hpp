@@ -68,6 +68,7 @@ struct DState {
     double* dt;           // [NRp]
     double* specrad;      // [NRp]
     double* part;         // [ntiles][8] per-tile partial sums of the residual monitor
+    double* jac;          // [10][NRp] Jacobi values of the residual smoother (allocated on first use)
 };
 
 }  // namespace e3d

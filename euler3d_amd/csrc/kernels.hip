@@ -323,6 +323,52 @@ __global__ void __launch_bounds__(kWG) k_update(Range rg, DMesh m, DState s, dou
     for (int v = 0; v < 5; ++v) s.W[(size_t)v * m.NA + c] = w[v];
 }
 
+// CentralIrsSmoother::smoothResidual (Solver/src/CentralIrsSmoother.cpp:12-183):
+// one Jacobi sweep of central implicit residual smoothing (epsilon = 0.8) on
+// conv and diss of real cells. Component q < 5 is conv variable q, q >= 5 diss
+// variable q - 5. As in the reference, the w-dissipation Jacobi value lands in
+// the v slot (:90) and the w slot is never set; it holds zero here.
+constexpr double kIrsEpsilon = 0.8;
+
+__device__ __forceinline__ double* irs_res(const DState& s, int NRp, int q) {
+    return q < 5 ? s.conv + (size_t)q * NRp : s.diss + (size_t)(q - 5) * NRp;
+}
+
+__global__ void __launch_bounds__(kWG) k_irs_jacobi(Range rg, DMesh m, DState s) {
+    const int c = rg.c0 + blockIdx.x * kWG + threadIdx.x;
+    if (c >= rg.c1 || m.nf[c] == 0) return;
+    const int na = m.nf[c];
+    const size_t N = m.NRp;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        if (q == 7) continue;
+        const int dst = q == 8 ? 7 : q;
+        s.jac[dst * N + c] = (irs_res(s, m.NRp, q)[c] + 0.) / (1. + kIrsEpsilon * na);
+    }
+    s.jac[8 * N + c] = 0.0;
+}
+
+__global__ void __launch_bounds__(kWG) k_irs_apply(Range rg, DMesh m, DState s) {
+    const int c = rg.c0 + blockIdx.x * kWG + threadIdx.x;
+    if (c >= rg.c1 || m.nf[c] == 0) return;
+    const int na = m.nf[c];
+    const size_t N = m.NRp;
+    double sum[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) sum[q] = 0.;
+    for (int k = 0; k < na; ++k) {
+        const int nb = m.nbr[(size_t)k * N + c];
+        if (nb < m.NRp)  // real neighbours only (ghosts are numbered after every real cell)
+#pragma unroll
+            for (int q = 0; q < 10; ++q) sum[q] += s.jac[q * N + nb];
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        double* r = irs_res(s, m.NRp, q);
+        r[c] = (r[c] + kIrsEpsilon * sum[q]) / (1. + kIrsEpsilon * na);
+    }
+}
+
 __device__ __forceinline__ void apply_bc(const DMesh& m, int kind, int f, double gamma, const Free& fs,
                                          const double wi[5], double g[5]) {
     const double area = m.area[f];
@@ -495,6 +541,11 @@ void launch_flux_diss(const Range& rg, const DMesh& m, const DState& s, int flux
 }
 void launch_update(const Range& rg, const DMesh& m, const DState& s, double alpha, double gm1, hipStream_t st) {
     if (rg.c1 > rg.c0) hipLaunchKernelGGL(k_update, RGRID, rg, m, s, alpha, gm1);
+}
+void launch_smooth(const Range& rg, const DMesh& m, const DState& s, hipStream_t st) {
+    if (rg.c1 <= rg.c0) return;
+    hipLaunchKernelGGL(k_irs_jacobi, RGRID, rg, m, s);
+    hipLaunchKernelGGL(k_irs_apply, RGRID, rg, m, s);
 }
 void launch_bc(const Range& rg, const DMesh& m, const DState& s, double gamma, const double fs[5], hipStream_t st) {
     if (rg.g1 <= rg.g0) return;

@@ -217,6 +217,10 @@ class NoSmoother : public ResidualSmoother {
    public:
     void smoothResidual(Block*) override {}
 };
+class CentralIrsSmoother : public ResidualSmoother {  // Solver/src/CentralIrsSmoother.cpp:12-183
+   public:
+    void smoothResidual(Block* block) override;
+};
 
 // Solver/head/ResidualCalculator.h
 class ResidualCalculator {

@@ -234,12 +234,11 @@ void ResidualCalculator::setFluxScheme(double gamma, std::string choice) {
     else if (choice == "ausm") flux_scheme_ = new AusmScheme(gamma);
     else throw Euler3dError(E3D_EINVAL, "unknown flux scheme '" + choice + "'");  // the reference leaves a null scheme
 }
-void ResidualCalculator::setResidualSmoother(bool choice) {
+void ResidualCalculator::setResidualSmoother(bool choice) {  // Solver/src/ResidualCalculator.cpp:90-95
     delete residual_smoother_;
-    residual_smoother_ = nullptr;
-    if (choice) throw Euler3dError(E3D_EINVAL, "residual smoothing (CentralIrsSmoother) is not supported");
-    residual_smoother_ = new NoSmoother();
+    residual_smoother_ = choice ? (ResidualSmoother*)new CentralIrsSmoother() : (ResidualSmoother*)new NoSmoother();
 }
+void CentralIrsSmoother::smoothResidual(Block* b) { e3d_check(e3d_smooth_residual(b->ctx_, b->block_id_), b->ctx_); }
 // Solver/src/ResidualCalculator.cpp:24-41; `10^-16` is an integer XOR (-6), so the
 // dissipation always runs, as in the reference.
 void ResidualCalculator::computeResidual(Block* b) {

@@ -84,7 +84,7 @@ typedef struct e3d_params {
     int stages;           /* RK stages 1..5, coefficient tables of Solver/src/RungeKutta.cpp:23-68 */
     int max_iter;
     double criterion;     /* stop when log10 density residual drop <= criterion */
-    int smoothing;        /* residual smoothing: only 0 (NoSmoother) is supported */
+    int smoothing;        /* 0 NoSmoother, 1 CentralIrsSmoother (Solver/src/ResidualCalculator.cpp:90-95) */
     int flux;             /* E3D_FLUX_* */
     int order;            /* 1 or 2 */
     int gradient;         /* E3D_GRAD_* */
@@ -161,7 +161,12 @@ int e3d_flux_conv(e3d_ctx* ctx, int block, int scheme);
 /* FluxScheme::computeFluxDiss with current_beta_ = beta (Solver/head/FluxScheme.h:19):
  * RoeScheme.cpp:12-311 or AusmScheme.cpp:12-243 */
 int e3d_flux_diss(e3d_ctx* ctx, int block, int scheme, double beta);
-/* ResidualSmoother::smoothResidual (Solver/head/ResidualSmoother.h:15): NoSmoother no-op */
+/* CentralIrsSmoother::smoothResidual (Solver/src/CentralIrsSmoother.cpp:12-183): one central
+ * implicit-smoothing Jacobi sweep on conv and diss of the block's real cells, including the
+ * reference's use of the w-dissipation value in the v sweep (:90); the w sweep, whose Jacobi
+ * array the reference never writes, smooths with zero. NoSmoother is no call at all.
+ * e3d_residual calls it when params.smoothing == 1; e3d_iterate then runs the operators
+ * block by block instead of the fused stage kernels. */
 int e3d_smooth_residual(e3d_ctx* ctx, int block);
 /* ResidualCalculator::computeResidual (Solver/src/ResidualCalculator.cpp:24-41) for RK stage `stage` */
 int e3d_residual(e3d_ctx* ctx, int block, int stage);

@@ -33,7 +33,7 @@ class Params(C.Structure):
                 ("mach", C.c_double), ("cmac", C.c_double), ("omega", C.c_double),
                 ("k", C.c_double), ("criterion", C.c_double), ("stages", C.c_int),
                 ("order", C.c_int), ("flux", C.c_int), ("gradient", C.c_int),
-                ("limiter", C.c_int), ("max_iter", C.c_int)]
+                ("limiter", C.c_int), ("max_iter", C.c_int), ("smoothing", C.c_int)]
 
 
 class Interface(C.Structure):
@@ -129,9 +129,9 @@ class OracleBlock:
 
 
 def params(mach, aoa_deg=0.0, flux=0, order=1, gradient=0, limiter=1, omega=1e-16, k=5.0, gamma=1.4,
-           cfl=1.5, stages=5, cmac=1.0, max_iter=200, criterion=-8.0):
+           cfl=1.5, stages=5, cmac=1.0, max_iter=200, criterion=-8.0, smoothing=0):
     return Params(gamma, cfl, aoa_deg, mach, cmac, omega, k, criterion, stages, order, flux, gradient,
-                  limiter, max_iter)
+                  limiter, max_iter, smoothing)
 
 
 def freestream(p):

@@ -7,6 +7,7 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 #define NV 5
 
@@ -561,6 +562,46 @@ void e3do_rk_coefficients(int stages, double a[5], double bt[5]) {
 
 /* ResidualCalculator::computeResidual (Solver/src/ResidualCalculator.cpp:24-41); the
  * dissipation always runs because `10^-16` is an integer XOR (= -6). */
+/* CentralIrsSmoother::smoothResidual (Solver/src/CentralIrsSmoother.cpp:12-183):
+ * one Jacobi sweep of central implicit residual smoothing, epsilon = 0.8, over
+ * real cells and their real neighbours, on conv and diss. Kept as the
+ * reference computes it: the dissipative Jacobi value of w is stored into the
+ * v array (:90), so the v sweep reads w values, and the w array is never
+ * written; it is read here as zero (a freshly zeroed allocation). */
+void e3do_smooth_residual(e3do_block* b) {
+    const int n = b->nreal;
+    const double epsilon = 0.8;
+    double* jac = (double*)calloc((size_t)10 * n, sizeof(double));
+    double* out = (double*)malloc(sizeof(double) * 10 * (size_t)n);
+    double* res[10];
+    for (int v = 0; v < 5; ++v) {
+        res[v] = b->conv + (size_t)v * b->nall;
+        res[5 + v] = b->diss + (size_t)v * b->nall;
+    }
+    for (int c = 0; c < n; ++c) {
+        const int na = b->cell_off[c + 1] - b->cell_off[c];
+        for (int q = 0; q < 10; ++q) {
+            const int dst = (q == 8) ? 7 : q;          /* w diss -> v diss slot (:90) */
+            if (q == 7) continue;                      /* v diss: overwritten by w diss */
+            jac[(size_t)dst * n + c] = (res[q][c] + 0.) / (1. + epsilon * na);
+        }
+    }
+    for (int c = 0; c < n; ++c) {
+        double sum[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+        const int na = b->cell_off[c + 1] - b->cell_off[c];
+        for (int j = b->cell_off[c]; j < b->cell_off[c + 1]; ++j) {
+            const int nb = b->cell2cell[j];
+            if (nb < n)
+                for (int q = 0; q < 10; ++q) sum[q] += jac[(size_t)q * n + nb];
+        }
+        for (int q = 0; q < 10; ++q) out[(size_t)q * n + c] = (res[q][c] + epsilon * sum[q]) / (1. + epsilon * na);
+    }
+    for (int q = 0; q < 10; ++q)
+        for (int c = 0; c < n; ++c) res[q][c] = out[(size_t)q * n + c];
+    free(jac);
+    free(out);
+}
+
 void e3do_residual(e3do_block* b, const e3do_params* p, int stage) {
     double a[5], bt[5];
     e3do_rk_coefficients(p->stages, a, bt);
@@ -577,6 +618,7 @@ void e3do_residual(e3do_block* b, const e3do_params* p, int stage) {
         e3do_flux_conv_central(b, p->gamma);
         e3do_flux_diss_roe(b, p->gamma, bt[stage]);
     }
+    if (p->smoothing) e3do_smooth_residual(b);
 }
 
 /* RungeKutta::computeRungeKutta (Solver/src/RungeKutta.cpp:80-94) */

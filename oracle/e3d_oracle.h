@@ -35,6 +35,7 @@ typedef struct e3do_block {
 typedef struct e3do_params {
     double gamma, cfl, aoa_deg, mach, cmac, omega, k, criterion;
     int stages, order, flux, gradient, limiter, max_iter;   /* flux 0 roe 1 ausm; gradient 0 GG 1 LSQ; limiter 0 BJ 1 VK */
+    int smoothing;                                          /* 1: CentralIrsSmoother after each residual */
 } e3do_params;
 
 typedef struct e3do_interface {        /* one CONNEXION boundary: ghosts of dst filled from origin cells */
@@ -63,6 +64,7 @@ void e3do_flux_diss_ausm(e3do_block* b, double gamma, double beta);
 void e3do_update(e3do_block* b, double gamma, double alpha);
 void e3do_bc(e3do_block* b, const double fs[5], double gamma);
 void e3do_rk_coefficients(int stages, double alpha[5], double beta[5]);
+void e3do_smooth_residual(e3do_block* b);
 void e3do_residual(e3do_block* b, const e3do_params* p, int stage);
 void e3do_runge_kutta(e3do_block* b, const e3do_params* p, const double fs[5]);
 void e3do_convergence(const e3do_block* b, double rms[5]);

@@ -21,6 +21,8 @@
 #include "FarfieldCellIds.h"
 #include "SymmetryCellIds.h"
 #include "ConnexionBlockCellIds.h"
+#include <malloc.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -144,6 +146,11 @@ static void dump_state(Block* b, const std::string& tag) {
 
 int main(int argc, char* argv[]) {
     if (argc < 5) { fprintf(stderr, "usage: ref_dump input out_dir iters stage_iter\n"); return 2; }
+    // Every allocation starts as zero bytes (glibc fills it with the complement
+    // of 0xff). The reference reads one array it never writes, the w Jacobi
+    // array of CentralIrsSmoother (Solver/src/CentralIrsSmoother.cpp:90); this
+    // makes that read the zero a fresh allocation holds, on every call.
+    mallopt(M_PERTURB, 0xff);
     g_out = argv[2];
     mkdir(g_out.c_str(), 0755);
     std::vector<int> dump_iters;

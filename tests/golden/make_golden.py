@@ -73,6 +73,11 @@ CASES = {
                                omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
     "naca_hex_roe2_gg_vk": dict(mesh="naca", mach=0.8, aoa=1.25, flux=0, order=2, grad=0, lim=1,
                                 omega=1e-16, k=5.0, stages=5, iters=8, dumps=[4, 5], stage_iter=5),
+    # residual smoothing on (CentralIrsSmoother)
+    "bump1_roe2_gg_vk_irs": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1,
+                                 omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5, smooth=1),
+    "bump8_ausm1_irs": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=0.5, aoa=2.0, flux=1, order=1, grad=0, lim=1,
+                            omega=1e-16, k=5.0, stages=4, iters=12, dumps=[4, 5], stage_iter=5, smooth=1),
 }
 BIG = {
     # config 1 of the benchmark: 48x16x16x6 tets, first-order Roe, M=0.5, 200 iterations
@@ -84,7 +89,7 @@ BIG = {
 def input_file(path, topology, c):
     lines = ["MESH", "topologyfilename", topology, "INPUT", "cfl gamma angleattackdeg",
              f"1.5 1.4 {c['aoa']!r}", "rkstage mach cmac", f"{c['stages']} {c['mach']!r} 1.0",
-             "nbitermax convcriterion", f"{c['iters']} -15", "residual smoothing (0-no 1-yes)", "0",
+             "nbitermax convcriterion", f"{c['iters']} -15", "residual smoothing (0-no 1-yes)", str(c.get("smooth", 0)),
              "fluxscheme schemeorder", f"{FLUX[c['flux']]} {c['order']}", "gradient limiter",
              f"{GRAD[c['grad']]} {LIM[c['lim']]}", "omega k", f"{c['omega']!r} {c['k']!r}",
              "EXECUTABLE (0-no 1-yes)", "build execute", "1 1", "OUTPUT (0-no 1-yes)",

@@ -15,6 +15,7 @@ import oracle as O
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
 SMALL_FIXTURES = [f for f in FIXTURES if not f.startswith("c1_")]
+C1_CASE = "c1_roe1"  # 48x16x16x6 tets, first-order Roe, M=0.5, 200 iterations
 
 
 class Golden:
@@ -52,6 +53,14 @@ class Golden:
 
     def monitor(self):
         return self.z["monitor"].reshape(-1, 15)
+
+    def samples(self, key):
+        """(flat indices, reference values) of an array: all of it, or the
+        sampled values stored beside the digest of a large array."""
+        if key in self.z.files:
+            a = self.z[key].reshape(-1)
+            return np.arange(a.size), a
+        return self.z["i:" + key], self.z["s:" + key]
 
 
 def write_su2_from_fixture(g, path):
@@ -99,13 +108,14 @@ def mesh_for(g):
 def oracle_params(meta):
     return O.params(meta["mach"], meta["aoa"], flux=meta["flux"], order=meta["order"], gradient=meta["grad"],
                     limiter=meta["lim"], omega=meta["omega"], k=meta["k"], stages=meta["stages"],
-                    max_iter=meta["iters"], criterion=-15.0)
+                    max_iter=meta["iters"], criterion=-15.0, smoothing=meta.get("smooth", 0))
 
 
 def product_params(meta):
     return E.make_params(meta["mach"], meta["aoa"], flux=meta["flux"], order=meta["order"],
                          gradient=meta["grad"], limiter=meta["lim"], omega=meta["omega"], k=meta["k"],
-                         stages=meta["stages"], max_iter=meta["iters"], criterion=-15.0)
+                         stages=meta["stages"], max_iter=meta["iters"], criterion=-15.0,
+                         smoothing=meta.get("smooth", 0))
 
 
 def geometry_arrays(b):
@@ -126,6 +136,12 @@ def geometry_arrays(b):
 def real(a, nv, nall, nreal):
     """Real-cell part of a variable-major all-cell array."""
     return np.ascontiguousarray(a.reshape(nv, nall)[:, :nreal]).reshape(-1)
+
+
+def monitor_row(m):
+    """A monitor dict as the 15 doubles of the golden monitor rows."""
+    return np.array([m["iter"]] + list(m["rms"]) + [m["cl"], m["cd"], m["cmx"], m["cmy"], m["cmz"], m["drop"][0],
+                                                     m["drop"][1], m["drop"][4], m["stop"]], np.float64)
 
 
 def oracle_state(B, what):
@@ -150,7 +166,8 @@ def write_input(path, topology, meta, max_iter=None, criterion=-15.0):
     it = meta["iters"] if max_iter is None else max_iter
     lines = ["MESH", "topologyfilename", topology, "INPUT", "cfl gamma angleattackdeg",
              f"1.5 1.4 {meta['aoa']!r}", "rkstage mach cmac", f"{meta['stages']} {meta['mach']!r} 1.0",
-             "nbitermax convcriterion", f"{it} {criterion!r}", "residual smoothing (0-no 1-yes)", "0",
+             "nbitermax convcriterion", f"{it} {criterion!r}", "residual smoothing (0-no 1-yes)",
+             str(meta.get("smooth", 0)),
              "fluxscheme schemeorder", f"{FLUX_NAMES[meta['flux']]} {meta['order']}", "gradient limiter",
              f"{GRAD_NAMES[meta['grad']]} {LIM_NAMES[meta['lim']]}", "omega k", f"{meta['omega']!r} {meta['k']!r}",
              "EXECUTABLE (0-no 1-yes)", "build execute", "1 1", "OUTPUT (0-no 1-yes)", "clalpha cdalpha cmalpha",

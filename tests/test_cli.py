@@ -66,7 +66,7 @@ def expected_rows(g):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["bump8_roe2_gg_vk", "naca_hex_roe2_gg_vk", "bump1_ausm2_lsq_bj"])
+@pytest.mark.parametrize("case", ["bump8_roe2_gg_vk", "naca_hex_roe2_gg_vk", "bump1_ausm2_lsq_bj", "bump8_ausm1_irs"])
 @pytest.mark.parametrize("mode", ["fused", "granular"])
 def test_cli_convergence_history(tmp_path, case, mode):
     g = Golden(case)

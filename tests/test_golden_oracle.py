@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import SMALL_FIXTURES, Golden, mesh_for, oracle_params, oracle_state
+from helpers import C1_CASE, SMALL_FIXTURES, Golden, mesh_for, monitor_row, oracle_params, oracle_state
 
 
 def run_oracle_against(g):
@@ -47,8 +47,7 @@ def test_oracle_matches_reference(case):
     ref = g.monitor()
     assert len(rows) == len(ref)
     for r, m in zip(ref, rows):
-        mine = np.array([m["iter"]] + m["rms"] + [m["cl"], m["cd"], m["cmx"], m["cmy"], m["cmz"], m["drop"][0],
-                                                   m["drop"][1], m["drop"][4], m["stop"]])
+        mine = monitor_row(m)
         same = (mine.view(np.int64) == r.view(np.int64)) | (np.isnan(mine) & np.isnan(r))
         assert same.all(), f"iteration {int(r[0])}: monitor {mine[~same]} vs reference {r[~same]}"
     nb = len(rows[0]["block"])
@@ -72,3 +71,26 @@ def test_oracle_fused_loop_equals_hooked_loop():
         assert x["rms"] == y["rms"] and x["cl"] == y["cl"] and x["cmz"] == y["cmz"]
     for Ba, Bb in zip(a.blocks, b.blocks):
         assert (Ba.W.view(np.int64) == Bb.W.view(np.int64)).all()
+
+
+@pytest.mark.slow
+def test_oracle_c1_config_200_iterations():
+    """The 48x16x16x6-tet channel (73,728 cells), first-order Roe, M=0.5, 200
+    iterations from freestream: every array after the last iteration and the
+    whole monitor history, bit for bit against the reference run (about a
+    minute on one core)."""
+    g = Golden(C1_CASE)
+    meta = g.meta
+    mesh = mesh_for(g)
+    S = O.Solver([mesh.block(0)], oracle_params(meta))
+    S.init_freestream()
+    last = meta["iters"] - 1
+
+    def on_iter(it, blocks):
+        if it == last:
+            for what in ("W", "W0", "conv", "diss", "dt", "specrad", "grad", "lim"):
+                g.check(f"it{it}_b0_{what}", oracle_state(blocks[0], what), f"after iteration {it}")
+
+    rows = S.run(meta["iters"], on_iter=on_iter)
+    assert len(rows) == meta["iters"]
+    g.check("monitor", np.concatenate([monitor_row(m) for m in rows]), "monitor history")

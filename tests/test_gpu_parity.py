@@ -14,7 +14,7 @@ import pytest
 
 import euler3d_amd as E
 import oracle as O
-from helpers import SMALL_FIXTURES, Golden, mesh_for, oracle_params, product_params, values_equal
+from helpers import C1_CASE, SMALL_FIXTURES, Golden, mesh_for, monitor_row, oracle_params, product_params, values_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +67,55 @@ def test_granular_operators_match_reference_bitwise(case):
         if it in meta["dumps"]:
             for b in range(nb):
                 g.check(f"it{it}_b{b}_W", ctx.get(b, E.ARR_W).reshape(-1), f"GPU after iteration {it}")
+    ctx.close()
+
+
+def test_c1_config_granular_200_iterations_bitwise():
+    """The 48x16x16x6-tet channel (73,728 cells), first-order Roe, M=0.5, 200
+    iterations through the granular operators in Solver::solve order: every
+    array after the last iteration bit for bit against the reference run, and
+    the reference's sampled monitor rows within 1e-12."""
+    g = Golden(C1_CASE)
+    meta = g.meta
+    mesh = mesh_for(g)
+    ctx = gpu_context(mesh, product_params(meta))
+    print(f"c1 granular: {meta['iters']} iterations", flush=True)
+    ctx.op("exchange")
+    rows = []
+    for it in range(meta["iters"]):
+        ctx.op("save_w0", 0)
+        ctx.op("spectral_radius", 0)
+        ctx.op("timestep", 0)
+        ctx.op("runge_kutta", 0)
+        ctx.op("exchange")
+        rows.append(monitor_row(ctx.monitors()))
+    for what in ("W", "W0", "conv", "diss", "dt", "specrad", "grad", "lim"):
+        g.check(f"it{meta['iters'] - 1}_b0_{what}", ctx.get(0, ARR[what]).reshape(-1), "GPU granular")
+    idx, ref = g.samples("monitor")
+    mine = np.concatenate(rows)[idx]
+    assert close_rel(mine, ref), np.flatnonzero(~np.isclose(mine, ref, rtol=MON_RTOL, atol=0))[:8]
+    ctx.close()
+
+
+def test_c1_config_fused_200_iterations():
+    """The same run through e3d_iterate: sampled states after 200 iterations
+    equal to the reference's (as values), sampled monitors within 1e-12, and
+    the run stops at iteration 199 on max_iter as the reference did."""
+    g = Golden(C1_CASE)
+    meta = g.meta
+    mesh = mesh_for(g)
+    ctx = gpu_context(mesh, product_params(meta))
+    print(f"c1 fused: {meta['iters']} iterations", flush=True)
+    hist = ctx.iterate(meta["iters"] + 10)
+    assert len(hist) == meta["iters"] and hist[-1]["stop"] == 1
+    last = meta["iters"] - 1
+    for what in ("W", "W0", "conv", "diss", "dt"):
+        idx, ref = g.samples(f"it{last}_b0_{what}")
+        mine = ctx.get(0, ARR[what]).reshape(-1)[idx]
+        assert values_equal(mine, ref).all(), (what, idx[~values_equal(mine, ref)][:5])
+    idx, ref = g.samples("monitor")
+    mine = np.concatenate([monitor_row(m) for m in hist])[idx]
+    assert close_rel(mine, ref)
     ctx.close()
 
 
@@ -200,7 +249,7 @@ def test_abi_errors_on_device():
         ctx.set_params(p)
     assert e.value.code == -1
     p = E.make_params(0.5)
-    p.smoothing = 1
+    p.smoothing = 2
     with pytest.raises(E.E3DError):
         ctx.set_params(p)
     ctx.set_params(E.make_params(0.5))
