@@ -39,7 +39,8 @@ class BlockView(C.Structure):
                 ("volume", _dp), ("ccenter", _dp), ("lsqw", _dp),
                 ("bnd_type", _ip), ("bnd_off", _ip), ("bnd_ghosts", _ip),
                 ("wall_faces", _ip), ("ghost_count", C.c_int), ("n_interfaces", C.c_int),
-                ("itf_origin_block", _ip), ("itf_off", _ip), ("itf_origin_cells", _ip)]
+                ("itf_origin_block", _ip), ("itf_off", _ip), ("itf_origin_cells", _ip),
+                ("cell_node_off", _ip), ("cell_nodes", _ip), ("node_xyz", _dp)]
 
 
 class Params(C.Structure):
@@ -106,6 +107,8 @@ def lib():
             "e3d_iterate": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor)]),
             "e3d_request_stop": (C.c_int, [C.c_void_p]),
             "e3d_reset_monitors": (C.c_int, [C.c_void_p]),
+            "e3d_monitor_state": (C.c_int, [C.c_void_p, _ip, _dp]),
+            "e3d_set_monitor_state": (C.c_int, [C.c_void_p, C.c_int, _dp]),
             "e3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
             "e3d_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_sync": (C.c_int, [C.c_void_p]),
@@ -186,6 +189,9 @@ class Mesh:
         out["itf_origin_block"] = _arr(v.itf_origin_block, v.n_interfaces, np.int32)
         out["itf_off"] = _arr(v.itf_off, v.n_interfaces + 1, np.int32)
         out["itf_origin_cells"] = _arr(v.itf_origin_cells, int(out["itf_off"][-1]), np.int32)
+        out["cell_node_off"] = _arr(v.cell_node_off, NA + 1, np.int32)
+        out["cell_nodes"] = _arr(v.cell_nodes, int(out["cell_node_off"][-1]), np.int32)
+        out["node_xyz"] = _arr(v.node_xyz, 3 * v.nnode, np.float64)
         return out
 
     def close(self):

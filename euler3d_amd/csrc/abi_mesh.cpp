@@ -166,6 +166,9 @@ int e3d_mesh_block(const e3d_mesh* m, int b, e3d_block_view* v) {
     v->itf_origin_block = F.itf_origin_block.data();
     v->itf_off = F.itf_off.data();
     v->itf_origin_cells = F.itf_origin_cells.data();
+    v->cell_node_off = B.cell_nodes_off.data();
+    v->cell_nodes = B.cell_nodes.data();
+    v->node_xyz = m->raw.blocks[b].xyz.data();
     return E3D_OK;
 }
 

@@ -717,6 +717,21 @@ int e3d_reset_monitors(e3d_ctx* c) {
     return E3D_OK;
 }
 
+int e3d_monitor_state(const e3d_ctx* c, int* iter, double rms0[5]) {
+    if (!c || !iter || !rms0) return set_error(nullptr, E3D_EINVAL, "e3d_monitor_state: null argument");
+    *iter = c->iter;
+    for (int v = 0; v < 5; ++v) rms0[v] = c->rms0[v];
+    return E3D_OK;
+}
+
+int e3d_set_monitor_state(e3d_ctx* c, int iter, const double rms0[5]) {
+    if (!c || !rms0) return set_error(c, E3D_EINVAL, "e3d_set_monitor_state: null argument");
+    if (iter < 0) return set_error(c, E3D_EINVAL, "e3d_set_monitor_state: negative iteration");
+    c->iter = iter;
+    for (int v = 0; v < 5; ++v) c->rms0[v] = rms0[v];
+    return E3D_OK;
+}
+
 int e3d_request_stop(e3d_ctx* c) {
     if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
     c->stop_request.store(1);

@@ -47,6 +47,8 @@ class Block {
     int n_wall_faces_ = 0;
     BoundaryCellIds** block_boundary_cell_ids_ = nullptr;
 
+    e3d_block_view view_{};             // host mesh arrays of the block (nodes, connectivity, faces)
+
     explicit Block(int block_id) : block_id_(block_id) {}
     ~Block();
     // Host copy of one array (E3D_ARR_*), reference layout (variable-major).
@@ -271,6 +273,30 @@ class Timestep {
     void computeTimestep(Block* block);
 };
 
+// PostProcessing/head/OutputTecplot.h: the solution files written when the
+// solver stops, in the reference's text format (default-formatted streams),
+// and a restart file that holds the state (the reference's is empty).
+class OutputTecplot {
+   public:
+    double mach_aircraft_, aoa_deg_, gamma_;
+    std::string output_dir_ = "./bin/";
+    OutputTecplot(double mach_aircraft, double aoa_deg, double gamma)
+        : mach_aircraft_(mach_aircraft), aoa_deg_(aoa_deg), gamma_(gamma) {}
+    void printFlowData(Block* block);
+    void printSurfaceFlowData(Block* block);
+    void printConvergence(int iter, double cl, double cd, double cmx, double cmy, double cmz, double ro_convergence,
+                          double uu_convergence, double vv_convergence, double ww_convergence, double pp_convergence);
+    void printAerodynamicCoefficients(double cl, double cd, double cmx, double cmy, double cmz);
+    // Binary: "E3DRST01", int32 block, nreal, nall, iteration; double rms0[5];
+    // W [5][nall] (primitive, all cells); diss [5][nreal].
+    void printRestartFile(Block* block);
+    static std::string blockFileName(const std::string& stem, int block_id);
+};
+
+// Restores the state, iteration counter and reference residual written by
+// OutputTecplot::printRestartFile for every block of the mesh.
+void readRestartFiles(const std::string& directory, CompleteMesh* complete_mesh);
+
 // PostProcessing/head/PostProcessing.h: monitors, stop test and output files.
 class PostProcessing {
    public:
@@ -286,9 +312,12 @@ class PostProcessing {
            cmz_geometry_mesh_ = 0;
     double aoa_deg_;
     bool verbose_ = true;
+    bool write_solution_ = true;      // FlowData / SurfaceFlowData / RestartFile at the stop
     e3d_monitor last_{};
+    OutputTecplot* output_tecplot_ = nullptr;
     PostProcessing(int n_blocks, int max_iter, double convergence_criterion, double cmac, double mach_aircraft,
                    double aoa_deg, double gamma);
+    ~PostProcessing();
     void computeFlowData(Block*) {}  // per-block norms are computed on the device by process()
     void process(CompleteMesh* complete_mesh, BlockCommunicator* communicator);
     // One monitor row produced elsewhere (the fused path): stop test, log, files.

@@ -2,9 +2,13 @@
 // (input file → Interface → blocks → CompleteMesh → Solver::solve → elapsed
 // time) over the MI355X device path. Usage:
 //   euler3d <input_file> [--granular] [--quiet] [--device N] [--out DIR]
+//           [--restart DIR] [--no-solution]
 // --granular runs the operator-by-operator loop of Solver/src/Solver.cpp:76-98
-// instead of the stage-fused kernels (same results). Exit status 42 on an I/O
-// error (the reference's convention), 1 on any other library error.
+// instead of the stage-fused kernels (same results). --restart resumes from the
+// RestartFile<block>.dat files a previous run wrote into DIR when it stopped
+// (state, iteration counter and reference residual); --no-solution skips the
+// FlowData / SurfaceFlowData / RestartFile files at the stop. Exit status 42 on
+// an I/O error (the reference's convention), 1 on any other library error.
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -14,14 +18,16 @@
 
 int main(int argc, char* argv[]) {
     const char* input = nullptr;
-    bool granular = false, quiet = false;
+    bool granular = false, quiet = false, solution = true;
     int device = 0;
-    std::string out_dir = "./bin/";
+    std::string out_dir = "./bin/", restart_dir;
     for (int i = 1; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--granular")) granular = true;
         else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
         else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) device = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out_dir = std::string(argv[++i]) + "/";
+        else if (!std::strcmp(argv[i], "--restart") && i + 1 < argc) restart_dir = argv[++i];
+        else if (!std::strcmp(argv[i], "--no-solution")) solution = false;
         else if (!input) input = argv[i];
         else {
             std::cerr << "unexpected argument " << argv[i] << std::endl;
@@ -50,12 +56,14 @@ int main(int argc, char* argv[]) {
                                    device);
         delete[] my_blocks;
         complete_mesh.InitializeMyBlocks(&interface, &communicator);
+        if (!restart_dir.empty()) readRestartFiles(restart_dir, &complete_mesh);
         communicator.initializeBuffers();
         std::cout << "Initialisation du Solver" << std::endl << std::endl;
         Solver* solver = initializer.initializeSolver(&interface);
         solver->granular_ = granular;
         solver->post_processing_->verbose_ = !quiet;
         solver->post_processing_->output_dir_ = out_dir;
+        solver->post_processing_->write_solution_ = solution;
 
         const auto t_start = std::chrono::high_resolution_clock::now();
         solver->solve(&complete_mesh, &communicator);

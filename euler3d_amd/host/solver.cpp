@@ -5,6 +5,8 @@
 #include <sys/stat.h>
 
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -113,6 +115,7 @@ void CompleteMesh::InitializeMyBlocks(Interface* interface, BlockCommunicator*) 
         B->n_real_boundaries_in_block_ = v.n_real_boundaries;
         B->n_all_boundaries_in_block_ = v.n_boundaries;
         B->n_wall_faces_ = v.n_wall_faces;
+        B->view_ = v;
         B->block_boundary_cell_ids_ = new BoundaryCellIds*[v.n_boundaries];
         for (int j = 0; j < v.n_boundaries; ++j) {
             BoundaryCellIds* bc;
@@ -289,10 +292,182 @@ void RungeKutta::computeRungeKutta(Block* b) {  // Solver/src/RungeKutta.cpp:80-
 void Timestep::computeSpectralRadius(Block* b) { e3d_check(e3d_spectral_radius(b->ctx_, b->block_id_), b->ctx_); }
 void Timestep::computeTimestep(Block* b) { e3d_check(e3d_timestep(b->ctx_, b->block_id_), b->ctx_); }
 
+// ----------------------------------------------------------- OutputTecplot
+
+std::string OutputTecplot::blockFileName(const std::string& stem, int block_id) {
+    char id[16];
+    std::snprintf(id, sizeof id, "%04d", block_id);  // OUTPUT_ZERO_PADDING 4 (OutputTecplot.cpp:14)
+    return stem + id + ".dat";
+}
+
+namespace {
+
+// x, y and z of every node, one value per line (OutputTecplot.cpp:57-84).
+void print_nodes(std::ostream& o, const e3d_block_view& v) {
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < v.nnode; ++i) o << v.node_xyz[3 * i + d] << std::endl;
+}
+
+void print_zone_header(std::ostream& o, const char* title, int nnode, int nelem, const char* zonetype) {
+    o << "TTILE = \"" << title << "\"" << std::endl;
+    o << "VARIABLES=\"X\",\"Y\",\"Z\",\"RO\",\"UU\",\"VV\",\"WW\",\"PP\",\"CP\",\"MACH\"" << std::endl;
+    o << "ZONE T=\"FLOW_FIELD\"" << std::endl;
+    o << "Nodes=" << nnode << ", " << "Elements=" << nelem << ", " << "ZONETYPE=" << zonetype << std::endl;
+    o << "DATAPACKING=BLOCK" << std::endl;
+    o << "VARLOCATION=([4,5,6,7,8,9,10]=CELLCENTERED)" << std::endl;
+}
+
+std::ofstream open_output(const std::string& path) {
+    std::ofstream f(path);
+    if (!f.is_open()) throw Euler3dError(E3D_EIO, "Failed opening file " + path);
+    return f;
+}
+
+}  // namespace
+
+// OutputTecplot::printFlowData (PostProcessing/src/OutputTecplot.cpp:32-189)
+void OutputTecplot::printFlowData(Block* block) {
+    const e3d_block_view& v = block->view_;
+    const int nr = v.nreal, na = v.nall;
+    const std::vector<double> W = block->get(E3D_ARR_W);
+    const double *ro = &W[0], *uu = &W[(size_t)na], *vv = &W[2 * (size_t)na], *ww = &W[3 * (size_t)na],
+                 *pp = &W[4 * (size_t)na];
+    std::ofstream o = open_output(output_dir_ + blockFileName("FlowData_Block", block->block_id_));
+    print_zone_header(o, "Vizualisation of the volumetric solution", v.nnode, nr, "FEBRICK");
+    print_nodes(o, v);
+    for (const double* a : {ro, uu, vv, ww, pp})
+        for (int i = 0; i < nr; ++i) o << a[i] << std::endl;
+    const double dyn_head = 0.5 * gamma_ * mach_aircraft_ * mach_aircraft_;
+    for (int i = 0; i < nr; ++i) o << (pp[i] - 1.) / dyn_head << std::endl;
+    for (int i = 0; i < nr; ++i) {
+        const double a_cell = std::pow(gamma_ * pp[i] / ro[i], 0.5);
+        const double velocity = std::pow(std::pow(uu[i], 2) + std::pow(vv[i], 2) + std::pow(ww[i], 2), 0.5);
+        o << velocity / a_cell << std::endl;
+    }
+    for (int i = 0; i < nr; ++i) {
+        for (int j = v.cell_node_off[i]; j < v.cell_node_off[i + 1]; ++j) o << v.cell_nodes[j] + 1 << " ";
+        o << std::endl;
+    }
+}
+
+// OutputTecplot::printSurfaceFlowData (PostProcessing/src/OutputTecplot.cpp:191-411), wall faces
+// only, face values the mean of the interior cell and its ghost. The Mach number pairs the
+// interior pressure with the ghost's density and velocity, as the reference does (:377-381).
+void OutputTecplot::printSurfaceFlowData(Block* block) {
+    const e3d_block_view& v = block->view_;
+    if (v.n_wall_faces == 0) return;
+    const int na = v.nall, nw = v.n_wall_faces;
+    const std::vector<double> W = block->get(E3D_ARR_W);
+    auto var = [&](int q, int cell) { return W[(size_t)q * na + cell]; };
+    std::ofstream o = open_output(output_dir_ + blockFileName("SurfaceFlowData_Block", block->block_id_));
+    print_zone_header(o, "Vizualisation of the surface solution", v.nnode, nw, "FEQUADRILATERAL");
+    print_nodes(o, v);
+    for (int q = 0; q < 5; ++q)
+        for (int i = 0; i < nw; ++i) {
+            const int f = v.wall_faces[i], c0 = v.face2cell[2 * f], c1 = v.face2cell[2 * f + 1];
+            o << (var(q, c0) + var(q, c1)) * 0.5 << std::endl;
+        }
+    const double dyn_head = 0.5 * gamma_ * mach_aircraft_ * mach_aircraft_;
+    for (int i = 0; i < nw; ++i) {
+        const int f = v.wall_faces[i], c0 = v.face2cell[2 * f], c1 = v.face2cell[2 * f + 1];
+        o << ((var(4, c0) + var(4, c1)) * 0.5 - 1.) / dyn_head << std::endl;
+    }
+    for (int i = 0; i < nw; ++i) {
+        const int f = v.wall_faces[i], c0 = v.face2cell[2 * f], c1 = v.face2cell[2 * f + 1];
+        const double p0 = var(4, c0);
+        const double pp = (p0 + var(4, c1)) * 0.5, ro = (p0 + var(0, c1)) * 0.5, uu = (p0 + var(1, c1)) * 0.5,
+                     vv = (p0 + var(2, c1)) * 0.5, ww = (p0 + var(3, c1)) * 0.5;
+        const double a_face = std::pow(gamma_ * pp / ro, 0.5);
+        const double velocity = std::pow(std::pow(uu, 2) + std::pow(vv, 2) + std::pow(ww, 2), 0.5);
+        o << velocity / a_face << std::endl;
+    }
+    for (int i = 0; i < nw; ++i) {
+        const int f = v.wall_faces[i], nn = v.face_nnode[f];
+        for (int j = 0; j < nn; ++j) {
+            o << v.face_nodes[4 * f + j] + 1;
+            if (j == nn - 1) o << std::endl;
+            else o << " ";
+        }
+    }
+}
+
+// OutputTecplot::printConvergence (PostProcessing/src/OutputTecplot.cpp:413-446)
+void OutputTecplot::printConvergence(int iter, double cl, double cd, double cmx, double cmy, double cmz,
+                                     double ro_convergence, double uu_convergence, double vv_convergence,
+                                     double ww_convergence, double pp_convergence) {
+    std::ofstream o(output_dir_ + "Convergence.dat", iter == 0 ? std::ios::trunc : std::ios::app);
+    if (!o.is_open()) throw Euler3dError(E3D_EIO, "Failed opening file " + output_dir_ + "Convergence.dat");
+    if (iter == 0)
+        o << "Iteration" << " " << "Cl" << " " << "Cd" << " " << "Cmx" << " " << "Cmy" << " " << "Cmz" << " "
+          << "Density_Convergence" << " " << "Uu_Convergence" << " " << "Vv_Convergence" << " " << "Ww_Convergence"
+          << " " << "Pressure_Convergence" << " " << std::endl;
+    o << iter << " " << cl << " " << cd << " " << cmx << " " << cmy << " " << cmz << " " << ro_convergence << " "
+      << uu_convergence << " " << vv_convergence << " " << ww_convergence << " " << pp_convergence << std::endl;
+}
+
+// OutputTecplot::printAerodynamicCoefficients (PostProcessing/src/OutputTecplot.cpp:448-473)
+void OutputTecplot::printAerodynamicCoefficients(double cl, double cd, double cmx, double cmy, double cmz) {
+    std::ofstream o = open_output(output_dir_ + "AerodynamicCoefficients.dat");
+    o << "Angle of attack" << " " << "Cl" << " " << "Cd" << " " << "Cmx" << " " << "Cmy" << " " << "Cmz" << std::endl;
+    o << aoa_deg_ << " " << cl << " " << cd << " " << cmx << " " << cmy << " " << cmz << std::endl;
+}
+
+namespace {
+const char kRestartMagic[8] = {'E', '3', 'D', 'R', 'S', 'T', '0', '1'};
+}
+
+void OutputTecplot::printRestartFile(Block* block) {
+    const e3d_block_view& v = block->view_;
+    int32_t head[4] = {block->block_id_, v.nreal, v.nall, 0};
+    double rms0[5];
+    e3d_check(e3d_monitor_state(block->ctx_, &head[3], rms0), block->ctx_);
+    const std::vector<double> W = block->get(E3D_ARR_W), D = block->get(E3D_ARR_DISS);
+    std::ofstream o(output_dir_ + blockFileName("RestartFile", block->block_id_), std::ios::binary);
+    if (!o.is_open()) throw Euler3dError(E3D_EIO, "Failed opening restart file of block " + std::to_string(head[0]));
+    o.write(kRestartMagic, 8);
+    o.write((const char*)head, sizeof head);
+    o.write((const char*)rms0, sizeof rms0);
+    o.write((const char*)W.data(), sizeof(double) * W.size());
+    o.write((const char*)D.data(), sizeof(double) * D.size());
+    if (!o) throw Euler3dError(E3D_EIO, "short write of restart file of block " + std::to_string(head[0]));
+}
+
+void readRestartFiles(const std::string& directory, CompleteMesh* mesh) {
+    const std::string dir = directory.empty() || directory.back() == '/' ? directory : directory + "/";
+    for (int b = 0; b < mesh->n_blocks_; ++b) {
+        Block* B = mesh->all_blocks_[b];
+        const std::string path = dir + OutputTecplot::blockFileName("RestartFile", b);
+        std::ifstream in(path, std::ios::binary);
+        if (!in.is_open()) throw Euler3dError(E3D_EIO, "cannot open restart file " + path);
+        char magic[8];
+        int32_t head[4];
+        double rms0[5];
+        in.read(magic, 8);
+        in.read((char*)head, sizeof head);
+        in.read((char*)rms0, sizeof rms0);
+        if (!in || std::memcmp(magic, kRestartMagic, 8) != 0)
+            throw Euler3dError(E3D_EIO, path + " is not a restart file");
+        if (head[0] != b || head[1] != B->view_.nreal || head[2] != B->view_.nall)
+            throw Euler3dError(E3D_EINVAL, path + " does not match block " + std::to_string(b) + " of the mesh");
+        std::vector<double> W(5 * (size_t)head[2]), D(5 * (size_t)head[1]);
+        in.read((char*)W.data(), sizeof(double) * W.size());
+        in.read((char*)D.data(), sizeof(double) * D.size());
+        if (!in) throw Euler3dError(E3D_EIO, "truncated restart file " + path);
+        B->set(E3D_ARR_W, W);
+        B->set(E3D_ARR_DISS, D);
+        if (b == 0) e3d_check(e3d_set_monitor_state(mesh->ctx_, head[3], rms0), mesh->ctx_);
+    }
+}
+
 // ---------------------------------------------------------- PostProcessing
 
-PostProcessing::PostProcessing(int, int max_iter, double convergence_criterion, double, double, double aoa_deg, double)
-    : max_iter_(max_iter), convergence_criterion_(convergence_criterion), aoa_deg_(aoa_deg) {}
+PostProcessing::PostProcessing(int, int max_iter, double convergence_criterion, double, double mach_aircraft,
+                               double aoa_deg, double gamma)
+    : max_iter_(max_iter), convergence_criterion_(convergence_criterion), aoa_deg_(aoa_deg) {
+    output_tecplot_ = new OutputTecplot(mach_aircraft, aoa_deg, gamma);
+}
+
+PostProcessing::~PostProcessing() { delete output_tecplot_; }
 
 void PostProcessing::checkStopSolver() {  // PostProcessing/src/PostProcessing.cpp:59-83
     struct stat buffer;
@@ -305,11 +480,12 @@ void PostProcessing::process(CompleteMesh* complete_mesh, BlockCommunicator* com
     int rc = e3d_monitors(complete_mesh->ctx_, &last_);
     if (rc < 0 && rc != E3D_ENONFINITE) e3d_check(rc, complete_mesh->ctx_);
     (void)communicator;
-    record(last_, complete_mesh);
     if (rc == E3D_ENONFINITE) stop_solver_ = true;
+    record(last_, complete_mesh);
 }
 
-void PostProcessing::record(const e3d_monitor& m, CompleteMesh*) {
+// PostProcessing::process (PostProcessing/src/PostProcessing.cpp:222-308) for one monitor row.
+void PostProcessing::record(const e3d_monitor& m, CompleteMesh* complete_mesh) {
     last_ = m;
     current_iter_ = m.iter;
     ro_rms_mesh_ = m.rms[0]; uu_rms_mesh_ = m.rms[1]; vv_rms_mesh_ = m.rms[2]; ww_rms_mesh_ = m.rms[3]; pp_rms_mesh_ = m.rms[4];
@@ -326,19 +502,23 @@ void PostProcessing::record(const e3d_monitor& m, CompleteMesh*) {
         std::cout << "Iter: " << current_iter_ << " Convergence Ro: " << ro_convergence_ << " cl: " << cl_geometry_mesh_
                   << " cd: " << cd_geometry_mesh_ << std::endl;
     mkdir(output_dir_.c_str(), 0755);
-    // OutputTecplot::printConvergence (PostProcessing/src/OutputTecplot.cpp:413-446)
-    std::ofstream conv(output_dir_ + "Convergence.dat", current_iter_ == 0 ? std::ios::trunc : std::ios::app);
-    if (current_iter_ == 0)
-        conv << "Iteration Cl Cd Cmx Cmy Cmz Density_Convergence Uu_Convergence Vv_Convergence Ww_Convergence "
-                "Pressure_Convergence \n";
-    conv << current_iter_ << " " << cl_geometry_mesh_ << " " << cd_geometry_mesh_ << " " << cmx_geometry_mesh_ << " "
-         << cmy_geometry_mesh_ << " " << cmz_geometry_mesh_ << " " << ro_convergence_ << " " << uu_convergence_ << " "
-         << vv_convergence_ << " " << ww_convergence_ << " " << pp_convergence_ << "\n";
-    if (stop_solver_) {  // OutputTecplot::printAerodynamicCoefficients (:448-473)
-        std::ofstream aero(output_dir_ + "AerodynamicCoefficients.dat");
-        aero << "Angle of attack Cl Cd Cmx Cmy Cmz\n"
-             << aoa_deg_ << " " << cl_geometry_mesh_ << " " << cd_geometry_mesh_ << " " << cmx_geometry_mesh_ << " "
-             << cmy_geometry_mesh_ << " " << cmz_geometry_mesh_ << "\n";
+    output_tecplot_->output_dir_ = output_dir_;
+    output_tecplot_->printConvergence(current_iter_, cl_geometry_mesh_, cd_geometry_mesh_, cmx_geometry_mesh_,
+                                      cmy_geometry_mesh_, cmz_geometry_mesh_, ro_convergence_, uu_convergence_,
+                                      vv_convergence_, ww_convergence_, pp_convergence_);
+    if (stop_solver_) {
+        if (write_solution_ && complete_mesh) {
+            if (verbose_) std::cout << "Writing Solution......................................................" << std::endl;
+            for (int i = 0; i < complete_mesh->n_blocks_in_process_; ++i) {
+                Block* b = complete_mesh->all_blocks_[complete_mesh->my_blocks_[i]];
+                output_tecplot_->printFlowData(b);
+                output_tecplot_->printSurfaceFlowData(b);
+                output_tecplot_->printRestartFile(b);
+            }
+        }
+        output_tecplot_->printAerodynamicCoefficients(cl_geometry_mesh_, cd_geometry_mesh_, cmx_geometry_mesh_,
+                                                      cmy_geometry_mesh_, cmz_geometry_mesh_);
+        if (verbose_) std::cout << "========================END OF PROGRAM========================" << std::endl;
     }
     ++current_iter_;
 }

@@ -75,6 +75,8 @@ typedef struct e3d_block_view {
     int ghost_count;                                     /* topology GhostCount */
     int n_interfaces;
     const int *itf_origin_block, *itf_off, *itf_origin_cells;
+    const int *cell_node_off, *cell_nodes;               /* nall+1 CSR: element nodes in file order */
+    const double* node_xyz;                              /* 3*nnode, x y z per node */
 } e3d_block_view;
 
 /* Solver parameters: the numeric content of the reference's input file
@@ -199,6 +201,12 @@ int e3d_iterate(e3d_ctx* ctx, int n, e3d_monitor* history);
 int e3d_request_stop(e3d_ctx* ctx);
 /* Resets the iteration counter and the reference residual of the drop. */
 int e3d_reset_monitors(e3d_ctx* ctx);
+/* Iteration counter and iteration-0 residual norms (PostProcessing's current_iter_ and
+ * ro_rms0_mesh_ ... pp_rms0_mesh_, PostProcessing/src/PostProcessing.cpp:86-123): read them
+ * to write a restart file, set them to resume a run from one (no counterpart in the
+ * reference, whose restart file is empty: PostProcessing/src/OutputTecplot.cpp:475-495). */
+int e3d_monitor_state(const e3d_ctx* ctx, int* iter, double rms0[5]);
+int e3d_set_monitor_state(e3d_ctx* ctx, int iter, const double rms0[5]);
 
 /* --------------------------------------------------------- measurement */
 /* Average duration in ms of the named kernel over the last e3d_iterate call

@@ -14,6 +14,9 @@ then packs the dumped arrays:
                   cells; W0, dt, spectral radius at s = 0)
   monitor         rows (iter, rms x5, cl, cd, cmx, cmy, cmz, drop ro, drop u, drop p, stop)
   blockmon_it<i>_b<b>  per-block rms x5 and cl, cd, cm x3 for i < 3
+  out:<file>, outlen:<file>  SHA-256 and length of each text file the reference
+                  wrote into ./bin when it stopped (convergence history,
+                  coefficients, volume and surface Tecplot files)
 
 Arrays of up to 512 entries are stored as they are. A larger array X is stored
 as its SHA-256 digest (h:X, of its little-endian bytes) plus the values at 48
@@ -179,6 +182,18 @@ def run_case(name, c, outdir):
                 pack(arrays, "g" + base[len("geom_b"):], a)
             else:
                 pack(arrays, base, a)
+        # the reference's own output files at the stop (Convergence.dat, AerodynamicCoefficients.dat,
+        # FlowData_Block*.dat, SurfaceFlowData_Block*.dat): SHA-256 of their bytes and their length
+        import hashlib
+        outs = []
+        for fn in sorted(os.listdir(os.path.join(tmp, "bin"))):
+            if fn.startswith("RestartFile"):
+                continue  # empty in the reference
+            data = open(os.path.join(tmp, "bin", fn), "rb").read()
+            arrays["out:" + fn] = np.frombuffer(hashlib.sha256(data).digest(), np.uint8)
+            arrays["outlen:" + fn] = np.array([len(data)], np.int64)
+            outs.append(fn)
+        meta["outputs"] = outs
         meta["monitor_cols"] = ["iter", "rms_ro", "rms_u", "rms_v", "rms_w", "rms_p", "cl", "cd", "cmx",
                                 "cmy", "cmz", "drop_ro", "drop_u", "drop_p", "stop"]
         meta["mesh"] = list(meta["mesh"]) if meta["mesh"] != "naca" else "naca"

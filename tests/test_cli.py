@@ -1,10 +1,13 @@
 """The euler3d command line (main.cpp:17-101 of the reference) and the C++ host
-classes behind it: exit codes, and on the GPU, Convergence.dat and
-AerodynamicCoefficients.dat against the reference's own printed histories.
+classes behind it: exit codes, and on the GPU, every file the reference wrote
+when it stopped (Convergence.dat, AerodynamicCoefficients.dat, the volume and
+surface Tecplot files of each block) byte for byte, and a run resumed from the
+restart files against the uninterrupted reference run.
 
-The reference prints both files through a default-formatted std::ostream
-(PostProcessing/src/OutputTecplot.cpp:413-473): 6 significant digits, the
-same text as C printf's %g, so the rows are compared as text."""
+The reference prints its files through default-formatted std::ostreams
+(PostProcessing/src/OutputTecplot.cpp:32-473): 6 significant digits, the same
+text as C printf's %g, so the convergence rows are also compared as text."""
+import hashlib
 import math
 import os
 import subprocess
@@ -86,6 +89,55 @@ def test_cli_convergence_history(tmp_path, case, mode):
     last = g.monitor()[-1]
     assert aero[0] == "Angle of attack Cl Cd Cmx Cmy Cmz"
     assert aero[1] == " ".join("%g" % v for v in [g.meta["aoa"]] + list(last[6:11]))
+    assert_reference_outputs(g, out, g.meta["outputs"])
+
+
+def assert_reference_outputs(g, out, names):
+    """Each named output file equals the reference's, byte for byte."""
+    assert names
+    for fn in names:
+        data = (out / fn).read_bytes()
+        ref_len = int(g.z["outlen:" + fn][0])
+        assert len(data) == ref_len and hashlib.sha256(data).digest() == bytes(g.z["out:" + fn]), \
+            f"{fn}: {len(data)} bytes vs the reference's {ref_len}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "granular"])
+def test_cli_restart_resumes_bitwise(tmp_path, mode):
+    """Stop after 6 of 12 iterations, resume from the restart files: the
+    solution files equal those of the reference's uninterrupted 12-iteration
+    run byte for byte, and the convergence rows continue at iteration 6."""
+    g = Golden("bump8_roe2_gg_vk")
+    top = write_case_mesh(g, str(tmp_path))
+    first, second = tmp_path / "first", tmp_path / "second"
+    extra = ["--granular"] if mode == "granular" else []
+    write_input(tmp_path / "input6", top, g.meta, max_iter=6)
+    print(f"cli restart {mode}: first half", flush=True)
+    r = run_cli(str(tmp_path / "input6"), "--quiet", "--out", str(first), *extra, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert all((first / f"RestartFile{b:04d}.dat").exists() for b in range(8))
+    write_input(tmp_path / "input12", top, g.meta)
+    print(f"cli restart {mode}: second half", flush=True)
+    r = run_cli(str(tmp_path / "input12"), "--quiet", "--out", str(second), "--restart", str(first), *extra,
+                cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = (second / "Convergence.dat").read_text().splitlines()
+    assert rows == expected_rows(g)[6:]
+    assert_reference_outputs(g, second, [f for f in g.meta["outputs"] if f != "Convergence.dat"])
+
+
+@pytest.mark.gpu
+def test_cli_restart_rejects_foreign_file(tmp_path):
+    """A restart file that does not belong to the mesh is an I/O error (exit 42)."""
+    g = Golden("bump1_roe1")
+    top = write_case_mesh(g, str(tmp_path))
+    write_input(tmp_path / "input", top, g.meta)
+    bad = tmp_path / "bad"
+    bad.mkdir()
+    (bad / "RestartFile0000.dat").write_bytes(b"not a restart file")
+    r = run_cli(str(tmp_path / "input"), "--quiet", "--restart", str(bad), cwd=str(tmp_path))
+    assert r.returncode == 42 and "not a restart file" in r.stderr, r.stdout + r.stderr
 
 
 @pytest.mark.gpu
