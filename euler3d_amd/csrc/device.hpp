@@ -47,7 +47,8 @@ struct DMesh {
 
 // Face-state records of the fused path: record k*NRp + c holds the
 // reconstructed primitive state of real cell c on its k-th face, record
-// nslot*NRp + i that of ghost i on its face. Five doubles per record.
+// nslot*NRp + i that of ghost i on its face; five variables per record, stored
+// in three planes: (rho, u) pairs, (v, w) pairs, p.
 __device__ __host__ inline size_t ghost_record(int nslot, int NRp, int i) { return (size_t)nslot * NRp + i; }
 
 // Cell range of a granular launch: real cells [c0, c1) (global), ghosts
@@ -64,7 +65,7 @@ struct DState {
     double* diss;         // [5][NRp]
     double* grad;         // [15][NA]
     double* lim;          // [5][NA]
-    double* Wf;           // [nslot*NRp + NG][5] reconstructed face states (fused path)
+    double* Wf;           // [5 * (nslot*NRp + NG)] reconstructed face states (fused path), three planes
     double* dt;           // [NRp]
     double* specrad;      // [NRp]
     double* part;         // [ntiles][8] per-tile partial sums of the residual monitor

@@ -6,7 +6,7 @@
 //              each of its faces, written as one 5-double record per
 //              (cell, face).
 //   k_flux   : per cell and face, the flux from the cell's own face-state
-//              record and the other cell's record (one 40-byte gather),
+//              record and the other cell's record (one gathered record),
 //              accumulated in the reference's face order; then the RK update
 //              and the ghost BCs, into the second state buffer. On stage 0
 //              also saveW0, the spectral radius and the local time step.
@@ -28,16 +28,31 @@ __device__ __forceinline__ double slotv(const double* a, int q, int nslot, int N
     return a[((size_t)q * nslot + k) * NRp + c];
 }
 
-__device__ __forceinline__ void load_rec(const double* Wf, size_t rec, double w[5]) {
-    const double* p = Wf + rec * 5;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) w[v] = p[v];
+// Face-state records in three planes so that a record is three loads: (rho, u)
+// and (v, w) as 16-byte pairs, p alone. Record r: pair q (q = 0, 1) at
+// Wf[2*q*nrec + 2*r], p at Wf[4*nrec + r]; the records of a wavefront's cells
+// on one face slot are contiguous in each plane.
+__device__ __forceinline__ size_t rec_count(const DMesh& m) { return (size_t)m.nslot * m.NRp + m.NG; }
+
+__device__ __forceinline__ void load_rec(const double* Wf, size_t nrec, size_t rec, double w[5]) {
+    const double2 a = reinterpret_cast<const double2*>(Wf)[rec];
+    const double2 b = reinterpret_cast<const double2*>(Wf + 2 * nrec)[rec];
+    w[0] = a.x;
+    w[1] = a.y;
+    w[2] = b.x;
+    w[3] = b.y;
+    w[4] = Wf[4 * nrec + rec];
 }
 
-__device__ __forceinline__ void store_rec(double* Wf, size_t rec, const double w[5]) {
-    double* p = Wf + rec * 5;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) p[v] = w[v];
+// Address of variable v of record r.
+__device__ __forceinline__ size_t rec_index(size_t nrec, size_t rec, int v) {
+    return v < 4 ? (v >> 1) * 2 * nrec + 2 * rec + (v & 1) : 4 * nrec + rec;
+}
+
+__device__ __forceinline__ void store_rec(double* Wf, size_t nrec, size_t rec, const double w[5]) {
+    reinterpret_cast<double2*>(Wf)[rec] = make_double2(w[0], w[1]);
+    reinterpret_cast<double2*>(Wf + 2 * nrec)[rec] = make_double2(w[2], w[3]);
+    Wf[4 * nrec + rec] = w[4];
 }
 
 // W + (lim*grad).r, accumulated per component as FluxScheme.cpp:127-134 does.
@@ -66,7 +81,7 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
 #pragma unroll
     for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
     load3(m.g_r, m.NG, gi, rg);
-    double* rec = s.Wf + ghost_record(ns, NRp, gi) * 5;
+    const size_t nrec = rec_count(m), rec = ghost_record(ns, NRp, gi);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         double gg[3];
@@ -92,7 +107,7 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
         double out = wg;
 #pragma unroll
         for (int d = 0; d < 3; ++d) out += lim * gg[d] * rg[d];
-        rec[v] = out;
+        s.Wf[rec_index(nrec, rec, v)] = out;
     }
 }
 
@@ -178,7 +193,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 #pragma unroll
             for (int d = 0; d < 3; ++d) r[d] = slotv(m.sR, d, ns, NRp, k, c);
             face_state(wc, lim, g, r, out);
-            store_rec(s.Wf, (size_t)k * NRp + c, out);
+            store_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, out);
             const int nb = m.nbr[(size_t)k * NRp + c];
             if (nb >= NRp) ghost_face_state<VENKAT>(m, s, k, c, nb, g, wc, eps2, omega);  // a ghost's volume is its interior cell's
         }
@@ -218,8 +233,8 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 const double norm = m.sA[(size_t)k * NRp + c];
                 if (FIRST) sr += (fabs((wc[1]) * n[0] + (wc[2]) * n[1] + (wc[3]) * n[2]) + (c_center)) * norm;
                 if (ORDER2) {
-                    load_rec(s.Wf, (size_t)k * NRp + c, own);
-                    load_rec(s.Wf, rec, oth);
+                    load_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, own);
+                    load_rec(s.Wf, rec_count(m), rec, oth);
                 } else {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) own[v] = wc[v];
