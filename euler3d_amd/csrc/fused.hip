@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     const double vol = m.vol[c];
     // pass 1: gradient and the neighbour extrema, faces in the reference's order
     double g[15], mx[5], mn[5];
+    unsigned ghost_slots = 0;  // bit k: the neighbour across face slot k is a ghost
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
 #pragma unroll
@@ -132,7 +133,9 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
             double wn[5];
-            load_w(s.W, m.NA, m.nbr[(size_t)k * NRp + c], wn);
+            const int nb = m.nbr[(size_t)k * NRp + c];
+            if (nb >= NRp) ghost_slots |= 1u << k;
+            load_w(s.W, m.NA, nb, wn);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 if (wn[v] > mx[v]) mx[v] = wn[v];
@@ -161,7 +164,8 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] /= vol;
     }
-    // pass 2: limiter over the faces
+    // pass 2: limiter over the faces; the r vectors are kept in LDS for pass 3
+    __shared__ double r_lds[MAXS * 3][kWG];
     const double eps2 = k3 * (vol);
     double lim[5];
 #pragma unroll
@@ -175,7 +179,10 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
         if (k < nf) {
             double r[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) r[d] = slotv(m.sR, d, ns, NRp, k, c);
+            for (int d = 0; d < 3; ++d) {
+                r[d] = slotv(m.sR, d, ns, NRp, k, c);
+                r_lds[k * 3 + d][threadIdx.x] = r[d];
+            }
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double d2 = 0.0;
@@ -191,11 +198,11 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
         if (k < nf) {
             double r[3], out[5];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) r[d] = slotv(m.sR, d, ns, NRp, k, c);
+            for (int d = 0; d < 3; ++d) r[d] = r_lds[k * 3 + d][threadIdx.x];
             face_state(wc, lim, g, r, out);
             store_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, out);
-            const int nb = m.nbr[(size_t)k * NRp + c];
-            if (nb >= NRp) ghost_face_state<VENKAT>(m, s, k, c, nb, g, wc, eps2, omega);  // a ghost's volume is its interior cell's
+            if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
+                ghost_face_state<VENKAT>(m, s, k, c, m.nbr[(size_t)k * NRp + c], g, wc, eps2, omega);
         }
 }
 
