@@ -61,6 +61,15 @@ class Monitor(C.Structure):
                 "cl": self.cl, "cd": self.cd, "cmx": self.cmx, "cmy": self.cmy, "cmz": self.cmz}
 
 
+class Instance(C.Structure):
+    _fields_ = [("mach", C.c_double), ("aoa_deg", C.c_double), ("cmac", C.c_double), ("criterion", C.c_double),
+                ("max_iter", C.c_int)]
+
+
+class BatchStatus(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("stopped", C.c_int), ("last", Monitor)]
+
+
 class E3DError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"euler3d_amd error {code}: {msg}")
@@ -112,6 +121,12 @@ def lib():
             "e3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
             "e3d_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_sync": (C.c_int, [C.c_void_p]),
+            "e3d_upload_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(Instance), C.c_int]),
+            "e3d_batch_size": (C.c_int, [C.c_void_p]),
+            "e3d_batch_blocks": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
+            "e3d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(BatchStatus)]),
+            "e3d_batch_history": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor), C.c_int]),
+            "e3d_set_tile_order": (C.c_int, [C.c_void_p, C.c_int]),
         }
         for name in ("e3d_save_w0", "e3d_spectral_radius", "e3d_timestep", "e3d_interpolation",
                      "e3d_smooth_residual", "e3d_bc", "e3d_runge_kutta"):
@@ -248,6 +263,42 @@ class Context:
             v = BlockView()
             _check(lib().e3d_mesh_block(mesh._h, b, C.byref(v)))
             self.sizes.append({"nreal": v.nreal, "nall": v.nall})
+
+    def upload_batch(self, meshes, instances):
+        """Uploads meshes as one batch; instances: dicts with mach, aoa_deg, cmac,
+        criterion, max_iter (one per mesh)."""
+        n = len(meshes)
+        hs = (C.c_void_p * n)(*[m._h.value for m in meshes])
+        ins = (Instance * n)(*[Instance(d["mach"], d.get("aoa_deg", 0.0), d.get("cmac", 1.0),
+                                        d.get("criterion", -8.0), d.get("max_iter", 1000)) for d in instances])
+        self._c(lib().e3d_upload_batch(self._h, hs, ins, n))
+        self.sizes = []
+        for mesh in meshes:
+            for b in range(mesh.nblocks):
+                v = BlockView()
+                _check(lib().e3d_mesh_block(mesh._h, b, C.byref(v)))
+                self.sizes.append({"nreal": v.nreal, "nall": v.nall})
+
+    def batch_blocks(self, i):
+        first, count = C.c_int(), C.c_int()
+        self._c(lib().e3d_batch_blocks(self._h, i, C.byref(first), C.byref(count)))
+        return list(range(first.value, first.value + count.value))
+
+    def batch_run(self, max_iterations, use_graph=True):
+        """Runs the batch; returns (iterations launched, per-instance status dicts)."""
+        n = lib().e3d_batch_size(self._h)
+        st = (BatchStatus * n)()
+        done = self._c(lib().e3d_batch_run(self._h, max_iterations, 1 if use_graph else 0, st))
+        return done, [{"iterations": x.iterations, "stopped": x.stopped, "last": x.last.as_dict()} for x in st]
+
+    def batch_history(self, i, max_rows):
+        """Monitor rows of instance i, at most max_rows (from iteration 0)."""
+        rows = (Monitor * max(max_rows, 1))()
+        n = self._c(lib().e3d_batch_history(self._h, i, rows, max_rows))
+        return [rows[k].as_dict() for k in range(n)]
+
+    def set_tile_order(self, xcd_contiguous):
+        self._c(lib().e3d_set_tile_order(self._h, 1 if xcd_contiguous else 0))
 
     def init_freestream(self):
         self._c(lib().e3d_init_freestream(self._h))

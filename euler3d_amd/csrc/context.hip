@@ -29,10 +29,15 @@ void launch_exchange(int, const int*, const int*, int, double*, double*, double*
 void launch_rms_partials(const DMesh&, const DState&, hipStream_t);
 void launch_smooth(const Range&, const DMesh&, const DState&, hipStream_t);
 void launch_block_monitors(int, const int*, const int*, const int*, const int*, const int*, const int*,
-                           const double*, const DMesh&, const double*, const double*, double, double*, hipStream_t);
+                           const double*, const DMesh&, const double*, const double*, const double*, double*,
+                           hipStream_t);
 void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
-void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, bool, const Gas&, double, double,
-                        const double*, double, double, hipStream_t);
+void launch_batch_reduce(int, const int*, const double*, const void*, int*, double*, int*, int*, void*, int, int*,
+                         hipStream_t);
+void launch_batch_freeze(const DMesh&, const int*, const int*, const double*, double*, hipStream_t);
+
+void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, bool, const Gas&, double, double, double,
+                        double, hipStream_t);
 }  // namespace e3d
 
 using namespace e3d;
@@ -70,6 +75,27 @@ struct e3d_ctx {
     double* d_fcenter = nullptr;
     double* d_mon = nullptr;   // [nb][16]
     double* h_mon = nullptr;   // pinned
+    // instances: one mesh is instance 0 with the conditions of params; a batch
+    // (e3d_upload_batch) has one instance per mesh with its own conditions
+    bool batch = false;
+    int n_inst = 1;
+    std::vector<e3d_instance> inst;
+    std::vector<int> blk_inst, inst_b0;  // block -> instance; first block of each instance, then nb
+    std::vector<double> inst_fs;         // [n_inst][5]
+    int *d_blk_inst = nullptr, *d_g_blk = nullptr, *d_inst_b0 = nullptr;
+    double *d_inst_fs = nullptr, *d_blk_qcmac = nullptr;
+    // batched-run state on the device
+    int *d_active = nullptr, *d_frozen = nullptr, *d_iter = nullptr, *d_nactive = nullptr;
+    double* d_rms0 = nullptr;
+    void* d_prm = nullptr;
+    void* d_hist = nullptr;
+    int hist_cap = 0;
+    int* h_nactive = nullptr;  // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double* graph_W = nullptr;   // W buffer the graph was captured with
+    bool capturing = false;
+    int tile_map = 1;            // tile order of the stage kernels (e3d_set_tile_order)
     // monitors
     int iter = 0;
     bool started = false;      // initial exchange done
@@ -101,11 +127,22 @@ const char* ctx_error_text(const e3d_ctx* ctx) { return ctx->err.empty() ? nullp
 
 namespace {
 
+void drop_graph(e3d_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_W = nullptr;
+}
+
 void free_device(e3d_ctx* c) {
+    drop_graph(c);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     if (c->h_mon) (void)hipHostFree(c->h_mon);
     c->h_mon = nullptr;
+    if (c->h_nactive) (void)hipHostFree(c->h_nactive);
+    c->h_nactive = nullptr;
     c->have_mesh = false;
 }
 
@@ -155,7 +192,7 @@ int launch_status(e3d_ctx* c) {
 }
 
 void ev_begin(e3d_ctx* c, int id, size_t* slot) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     if (c->ev_used == c->ev_pool.size()) {
         e3d_ctx::Ev e;
         (void)hipEventCreate(&e.a);
@@ -167,7 +204,7 @@ void ev_begin(e3d_ctx* c, int id, size_t* slot) {
     (void)hipEventRecord(c->ev_pool[*slot].a, c->stream);
 }
 void ev_end(e3d_ctx* c, size_t slot) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     (void)hipEventRecord(c->ev_pool[slot].b, c->stream);
 }
 // After a synchronisation: fold the recorded launches into the per-kernel totals.
@@ -188,9 +225,8 @@ void ev_collect(e3d_ctx* c) {
 int finish_monitor(e3d_ctx* c, e3d_monitor* out) {
     size_t ev = 0;
     ev_begin(c, KI_MONITORS, &ev);
-    const double dyn_head = 0.5 * c->params.gamma * c->params.mach * c->params.mach;
     launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
-                          c->d_fcenter, c->m, c->s.W, c->s.part, dyn_head * c->params.cmac, c->d_mon, c->stream);
+                          c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mon, c->stream);
     ev_end(c, ev);
     int rc = launch_status(c);
     if (rc) return rc;
@@ -238,6 +274,47 @@ int exchange(e3d_ctx* c, int copy_all) {
     ev_end(c, ev);
     return launch_status(c);
 }
+
+// Freestream state (PrimitiveVariables::calculateFreeVariables,
+// DataStructure/src/PrimitiveVariables.cpp:41-56).
+void freestream_state(double gamma, double mach, double aoa_deg, double fs[5]) {
+    const double pi = std::atan(1.0) * 4.0;
+    const double alpha_rad = aoa_deg * pi / 180.0;
+    const double cs = std::cos(alpha_rad), sn = std::sin(alpha_rad);
+    fs[0] = 1.0;
+    fs[1] = mach * std::sqrt(gamma) * cs;
+    fs[2] = mach * std::sqrt(gamma) * sn;
+    fs[3] = 0.0;
+    fs[4] = 1.0;
+}
+
+// Per-instance freestream, monitor scale and stop constants, on the host and the
+// device. One mesh: instance 0 takes its conditions from params.
+int sync_instances(e3d_ctx* c) {
+    if (!c->have_params || !c->have_mesh) return E3D_OK;
+    const e3d_params& p = c->params;
+    if (!c->batch) c->inst.assign(1, e3d_instance{p.mach, p.aoa_deg, p.cmac, p.criterion, p.max_iter});
+    c->inst_fs.assign(5 * (size_t)c->n_inst, 0.0);
+    std::vector<InstParam> prm(c->n_inst);
+    for (int i = 0; i < c->n_inst; ++i) {
+        const e3d_instance& q = c->inst[i];
+        freestream_state(p.gamma, q.mach, q.aoa_deg, &c->inst_fs[5 * (size_t)i]);
+        const double aoa_rad = q.aoa_deg * 4. * std::atan(1.) / 180;  // as the monitor sums of finish_monitor
+        prm[i] = InstParam{std::cos(aoa_rad), std::sin(aoa_rad), q.criterion, q.max_iter, 0};
+    }
+    for (int v = 0; v < 5; ++v) c->fs[v] = c->inst_fs[v];
+    std::vector<double> qcmac(c->nb);
+    for (int b = 0; b < c->nb; ++b) {
+        const e3d_instance& q = c->inst[c->blk_inst[b]];
+        qcmac[b] = 0.5 * p.gamma * q.mach * q.mach * q.cmac;
+    }
+    HIP_TRY(c, hipMemcpy(c->d_inst_fs, c->inst_fs.data(), sizeof(double) * c->inst_fs.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_blk_qcmac, qcmac.data(), sizeof(double) * qcmac.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_prm, prm.data(), sizeof(InstParam) * prm.size(), hipMemcpyHostToDevice));
+    return E3D_OK;
+}
+
+const double* block_fs(const e3d_ctx* c, int b) { return &c->inst_fs[5 * (size_t)c->blk_inst[b]]; }
 
 }  // namespace
 
@@ -297,28 +374,41 @@ int e3d_set_params(e3d_ctx* c, const e3d_params* p) {
         c->alpha[i] = A[p->stages - 1][i];
         c->beta[i] = B[p->stages - 1][i];
     }
-    // PrimitiveVariables::calculateFreeVariables (DataStructure/src/PrimitiveVariables.cpp:41-56)
-    const double pi = std::atan(1.0) * 4.0;
-    const double alpha_rad = p->aoa_deg * pi / 180.0;
-    const double cs = std::cos(alpha_rad), sn = std::sin(alpha_rad);
-    c->fs[0] = 1.0;
-    c->fs[1] = p->mach * std::sqrt(p->gamma) * cs;
-    c->fs[2] = p->mach * std::sqrt(p->gamma) * sn;
-    c->fs[3] = 0.0;
-    c->fs[4] = 1.0;
+    freestream_state(p->gamma, p->mach, p->aoa_deg, c->fs);
     c->gas.gamma = p->gamma;
     c->gas.g_over_gm1 = p->gamma / (p->gamma - 1.0);
     c->gas.gm1 = p->gamma - 1.0;
     c->have_params = true;
-    return E3D_OK;
+    drop_graph(c);
+    return sync_instances(c);
 }
 
-int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
-    if (!c || !mesh) return set_error(c, E3D_EINVAL, "e3d_upload_mesh: null argument");
+// Every block of every mesh, in upload order, with the instance it belongs to.
+struct BlockSrc {
+    const HostBlock* B;
+    const e3d_mesh::Flat* F;
+    int ghost_count;
+    int base;  // global index of the mesh's block 0
+    int inst;
+};
+
+static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes) {
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_device(c);
-    const int nb = (int)mesh->built.size();
+    std::vector<BlockSrc> src;
+    c->inst_b0.assign(1, 0);
+    for (size_t i = 0; i < meshes.size(); ++i) {
+        const int base = (int)src.size();
+        for (size_t b = 0; b < meshes[i]->built.size(); ++b)
+            src.push_back({&meshes[i]->built[b], &meshes[i]->flat[b], meshes[i]->raw.topo[b].ghost_count, base,
+                           (int)i});
+        c->inst_b0.push_back((int)src.size());
+    }
+    c->n_inst = (int)meshes.size();
+    const int nb = (int)src.size();
+    c->blk_inst.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) c->blk_inst[b] = src[b].inst;
     c->nb = nb;
     c->nreal.assign(nb, 0);
     c->nall.assign(nb, 0);
@@ -330,13 +420,13 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     long NRp = 0, NG = 0, NF = 0;
     int nslot = 1;
     for (int b = 0; b < nb; ++b) {
-        const HostBlock& B = mesh->built[b];
+        const HostBlock& B = *src[b].B;
         c->nreal[b] = B.nreal;
         c->nall[b] = B.nall;
         c->roff[b] = (int)NRp;
         c->goff[b] = (int)NG;
         c->foff[b] = (int)NF;
-        c->ghost_count[b] = mesh->raw.topo[b].ghost_count;
+        c->ghost_count[b] = src[b].ghost_count;
         NRp += (long)(B.nreal + kWG - 1) / kWG * kWG;
         NG += B.nall - B.nreal;
         NF += B.nface;
@@ -363,9 +453,11 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     const size_t NS = (size_t)nslot * NR;
     std::vector<double> sS(3 * NS, 0.0), sR(3 * NS, 0.0), sN(3 * NS, 0.0), sA(NS, 1.0), g_r(3 * (size_t)NG, 0.0);
     std::vector<int> mate(NS, 0);
+    std::vector<int> g_blk(NG, 0);
     for (int b = 0; b < nb; ++b) {
-        const HostBlock& B = mesh->built[b];
-        const auto& F = mesh->flat[b];
+        const HostBlock& B = *src[b].B;
+        const auto& F = *src[b].F;
+        for (int gl = B.nreal; gl < B.nall; ++gl) g_blk[c->goff[b] + (gl - B.nreal)] = b;
         tile0[b] = c->roff[b] / kWG;
         ntile[b] = (B.nreal + kWG - 1) / kWG;
         for (int t = 0; t < ntile[b]; ++t) wg_block[tile0[b] + t] = b;
@@ -448,7 +540,7 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
         // interfaces: destination ghosts numbered from GhostCount in topology order
         int k = 0;
         for (size_t i = 0; i < F.itf_origin_block.size(); ++i) {
-            const int o = F.itf_origin_block[i];
+            const int o = src[b].base + F.itf_origin_block[i];
             for (int q = F.itf_off[i]; q < F.itf_off[i + 1]; ++q, ++k) {
                 itf_dst.push_back(global_cell(c, b, c->ghost_count[b] + k));
                 itf_org.push_back(c->roff[o] + F.itf_origin_cells[q]);
@@ -471,13 +563,20 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
         (rc = upload(c, &c->d_wall_off, wall_off)) || (rc = upload(c, &c->d_wall_ghost, wall_ghost)) ||
         (rc = upload(c, &c->d_wall_face, wall_face)) || (rc = upload(c, &d_mate, mate)) ||
         (rc = upload(c, &d_sS, sS)) || (rc = upload(c, &d_sR, sR)) || (rc = upload(c, &d_sN, sN)) ||
-        (rc = upload(c, &d_sA, sA)) || (rc = upload(c, &d_gr, g_r))) {
+        (rc = upload(c, &d_sA, sA)) || (rc = upload(c, &d_gr, g_r)) || (rc = upload(c, &c->d_blk_inst, c->blk_inst)) ||
+        (rc = upload(c, &c->d_g_blk, g_blk)) || (rc = upload(c, &c->d_inst_b0, c->inst_b0)) ||
+        (rc = dalloc(c, &c->d_inst_fs, 5 * (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_blk_qcmac, (size_t)nb)) ||
+        (rc = dalloc(c, &c->d_active, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_frozen, (size_t)c->n_inst)) ||
+        (rc = dalloc(c, &c->d_iter, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_nactive, 1)) ||
+        (rc = dalloc(c, &c->d_rms0, 5 * (size_t)c->n_inst)) ||
+        (rc = dalloc(c, (char**)&c->d_prm, sizeof(InstParam) * (size_t)c->n_inst))) {
         free_device(c);
         return rc;
     }
     m.nf = d_nf; m.nbr = d_nbr; m.fid = d_fid; m.S = d_S; m.area = d_area; m.rL = d_rL; m.rR = d_rR;
     m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
     m.sS = d_sS; m.sR = d_sR; m.sN = d_sN; m.sA = d_sA; m.mate = d_mate; m.g_r = d_gr;
+    m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr; m.tile_map = c->tile_map;
     DState& s = c->s;
     s.jac = nullptr;
     if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
@@ -489,6 +588,7 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
         return rc;
     }
     hipError_t e = hipHostMalloc((void**)&c->h_mon, sizeof(double) * 16 * nb, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_nactive, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_device(c);
         return set_error(c, E3D_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
@@ -498,7 +598,14 @@ int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
     c->have_mesh = true;
     c->iter = 0;
     c->started = false;
-    return E3D_OK;
+    return sync_instances(c);
+}
+
+int e3d_upload_mesh(e3d_ctx* c, const e3d_mesh* mesh) {
+    if (!c || !mesh) return set_error(c, E3D_EINVAL, "e3d_upload_mesh: null argument");
+    c->batch = false;
+    c->inst.assign(1, e3d_instance{});
+    return upload_meshes(c, {mesh});
 }
 
 int e3d_nblocks(const e3d_ctx* c) { return c ? c->nb : E3D_EINVAL; }
@@ -506,10 +613,22 @@ int e3d_nblocks(const e3d_ctx* c) { return c ? c->nb : E3D_EINVAL; }
 int e3d_init_freestream(e3d_ctx* c) {
     if (!c || !c->have_mesh || !c->have_params) return set_error(c, E3D_ESTATE, "e3d_init_freestream: need mesh and params");
     const size_t NA = c->m.NA;
-    std::vector<double> w(5 * NA);  // one host buffer per variable, alive until the copies complete
-    for (int v = 0; v < 5; ++v) std::fill(w.begin() + v * NA, w.begin() + (v + 1) * NA, c->fs[v]);
+    std::vector<double> w(5 * NA, 0.0);  // one host buffer, alive until the copies complete
+    for (int b = 0; b < c->nb; ++b) {  // every cell of a block gets its instance's freestream
+        const double* fs = block_fs(c, b);
+        const Range r = block_range(c, b);
+        for (int v = 0; v < 5; ++v) {
+            std::fill(w.begin() + v * NA + r.c0, w.begin() + v * NA + r.c1, fs[v]);
+            std::fill(w.begin() + v * NA + c->m.NRp + r.g0, w.begin() + v * NA + c->m.NRp + r.g1, fs[v]);
+        }
+    }
     HIP_TRY(c, hipMemcpyAsync(c->s.W, w.data(), sizeof(double) * 5 * NA, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->s.Wn, w.data(), sizeof(double) * 5 * NA, hipMemcpyHostToDevice, c->stream));
+    // batched-run state: every instance active at iteration 0
+    std::vector<int> ones(c->n_inst, 1), zeros(c->n_inst, 0);
+    HIP_TRY(c, hipMemcpyAsync(c->d_active, ones.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_frozen, zeros.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_iter, zeros.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->iter = 0;
     c->started = false;
@@ -669,7 +788,8 @@ int e3d_update(e3d_ctx* c, int block, int stage) {
 int e3d_bc(e3d_ctx* c, int block) {
     int rc = check_ready(c, block);
     if (rc) return rc;
-    launch_bc(block_range(c, block), c->m, c->s, c->params.gamma, c->fs, c->stream);
+    for (int b = block < 0 ? 0 : block; b < (block < 0 ? c->nb : block + 1); ++b)
+        launch_bc(block_range(c, b), c->m, c->s, c->params.gamma, block_fs(c, b), c->stream);
     return launch_status(c);
 }
 
@@ -682,7 +802,7 @@ int e3d_bc_marker(e3d_ctx* c, int block, int marker) {
     rg.c0 = rg.c1 = 0;
     rg.g1 = rg.g0 + c->marker_off[block][marker + 1];
     rg.g0 = rg.g0 + c->marker_off[block][marker];
-    launch_bc(rg, c->m, c->s, c->params.gamma, c->fs, c->stream);  // kernels skip ghosts without a BC
+    launch_bc(rg, c->m, c->s, c->params.gamma, block_fs(c, block), c->stream);  // kernels skip ghosts without a BC
     return launch_status(c);
 }
 
@@ -704,6 +824,7 @@ int e3d_exchange(e3d_ctx* c) {
 int e3d_monitors(e3d_ctx* c, e3d_monitor* out) {
     int rc = check_ready(c, -1);
     if (rc) return rc;
+    if (c->batch) return set_error(c, E3D_ESTATE, "e3d_monitors: a batch has per-instance monitors (e3d_batch_history)");
     launch_rms_partials(c->m, c->s, c->stream);
     if ((rc = launch_status(c))) return rc;
     return finish_monitor(c, out);
@@ -773,6 +894,7 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
     if (rc) return rc;
     if (n < 0) return set_error(c, E3D_EINVAL, "negative iteration count");
     HIP_TRY(c, hipSetDevice(c->device));
+    if (c->batch) return set_error(c, E3D_ESTATE, "e3d_iterate: a batch advances with e3d_batch_run");
     if (c->params.smoothing) return iterate_granular(c, n, history);
     const e3d_params& p = c->params;
     const bool order2 = p.order == 2;
@@ -794,7 +916,7 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
             }
             ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
             launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, st == 0, c->gas,
-                               c->beta[st], c->alpha[st], c->fs, p.cfl, 1. / (p.gamma - 1.0), c->stream);
+                               c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->stream);
             ev_end(c, ev);
             if ((rc = launch_status(c))) return rc;
             std::swap(c->s.W, c->s.Wn);
@@ -808,6 +930,172 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
         if (mon.stop) break;
     }
     return done;
+}
+
+// ---------------------------------------------------------------- batches
+
+static_assert(sizeof(MonRow) == sizeof(e3d_monitor), "monitor row layout");
+
+int e3d_upload_batch(e3d_ctx* c, const e3d_mesh* const* meshes, const e3d_instance* inst, int n) {
+    if (!c || !meshes || !inst || n < 1) return set_error(c, E3D_EINVAL, "e3d_upload_batch: bad arguments");
+    std::vector<const e3d_mesh*> ms(meshes, meshes + n);
+    int cap = 1;
+    for (int i = 0; i < n; ++i) {
+        if (!ms[i]) return set_error(c, E3D_EINVAL, "e3d_upload_batch: null mesh " + std::to_string(i));
+        if (inst[i].max_iter < 1) return set_error(c, E3D_EINVAL, "e3d_upload_batch: max_iter must be >= 1");
+        cap = std::max(cap, std::min(inst[i].max_iter, 200000));
+    }
+    c->batch = true;
+    c->inst.assign(inst, inst + n);
+    int rc = upload_meshes(c, ms);
+    if (rc) return rc;
+    c->hist_cap = cap;
+    if ((rc = dalloc(c, (char**)&c->d_hist, sizeof(MonRow) * (size_t)n * cap))) {
+        free_device(c);
+        return rc;
+    }
+    c->m.active = c->d_active;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return E3D_OK;
+}
+
+int e3d_batch_size(const e3d_ctx* c) { return c ? c->n_inst : E3D_EINVAL; }
+
+int e3d_batch_blocks(const e3d_ctx* c, int i, int* first, int* count) {
+    if (!c || !first || !count || i < 0 || i >= c->n_inst || !c->have_mesh)
+        return set_error(nullptr, E3D_EINVAL, "e3d_batch_blocks: bad argument");
+    *first = c->inst_b0[i];
+    *count = c->inst_b0[i + 1] - c->inst_b0[i];
+    return E3D_OK;
+}
+
+// One iteration of a batch: the stage kernels over every active instance, the
+// interface exchange, per-block monitor rows, then the per-instance rows, stop
+// tests and freezing. Launches only, so it can be captured.
+static int batch_iteration(e3d_ctx* c) {
+    const e3d_params& p = c->params;
+    const bool order2 = p.order == 2;
+    int rc;
+    for (int st = 0; st < p.stages; ++st) {
+        size_t ev = 0;
+        if (order2) {
+            ev_begin(c, KI_INTERP, &ev);
+            launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, p.k * p.k * p.k, p.omega,
+                                c->stream);
+            ev_end(c, ev);
+        }
+        ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
+        launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, st == 0, c->gas,
+                           c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->stream);
+        ev_end(c, ev);
+        if ((rc = launch_status(c))) return rc;
+        std::swap(c->s.W, c->s.Wn);
+    }
+    if ((rc = exchange(c, 0))) return rc;
+    size_t ev = 0;
+    ev_begin(c, KI_MONITORS, &ev);
+    launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
+                          c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mon, c->stream);
+    launch_batch_reduce(c->n_inst, c->d_inst_b0, c->d_mon, c->d_prm, c->d_iter, c->d_rms0, c->d_active, c->d_frozen,
+                        c->d_hist, c->hist_cap, c->d_nactive, c->stream);
+    launch_batch_freeze(c->m, c->d_frozen, c->d_g_blk, c->s.W, c->s.Wn, c->stream);
+    ev_end(c, ev);
+    return launch_status(c);
+}
+
+static int batch_active(e3d_ctx* c, int* n_active) {
+    HIP_TRY(c, hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    *n_active = *c->h_nactive;
+    return E3D_OK;
+}
+
+static int batch_capture(e3d_ctx* c) {
+    drop_graph(c);
+    double* w_start = c->s.W;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = batch_iteration(c);
+    if (!rc) rc = batch_iteration(c);  // two iterations: an even number of buffer swaps
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(c, E3D_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (c->s.W != w_start) return set_error(c, E3D_EHIP, "batch graph: unbalanced state buffers");
+    c->graph = g;
+    HIP_TRY(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->graph_W = w_start;
+    return E3D_OK;
+}
+
+int e3d_batch_run(e3d_ctx* c, int max_it, int use_graph, e3d_batch_status* status) {
+    int rc = check_ready(c, -1);
+    if (rc) return rc;
+    if (!c->batch) return set_error(c, E3D_ESTATE, "e3d_batch_run: no batch uploaded (e3d_upload_batch)");
+    if (c->params.smoothing) return set_error(c, E3D_EINVAL, "e3d_batch_run: residual smoothing is not supported");
+    if (max_it < 0) return set_error(c, E3D_EINVAL, "e3d_batch_run: negative iteration count");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->started) {  // Solver.cpp:48
+        if ((rc = exchange(c, 0))) return rc;
+        c->started = true;
+    }
+    int done = 0, n_active = 1;
+    if (use_graph) {
+        constexpr int kReplays = 8;  // graph launches (two iterations each) between host checks
+        if (c->gexec && c->graph_W != c->s.W) drop_graph(c);
+        if (!c->gexec && max_it >= 2 && (rc = batch_capture(c))) return rc;
+        while (n_active > 0 && max_it - done >= 2) {
+            const int k = std::min(kReplays, (max_it - done) / 2);
+            for (int r = 0; r < k; ++r) HIP_TRY(c, hipGraphLaunch(c->gexec, c->stream));
+            done += 2 * k;
+            if ((rc = batch_active(c, &n_active))) return rc;
+        }
+    }
+    while (n_active > 0 && done < max_it) {
+        if ((rc = batch_iteration(c))) return rc;
+        ++done;
+        if ((rc = batch_active(c, &n_active))) return rc;
+    }
+    if (status) {
+        std::vector<int> it(c->n_inst), act(c->n_inst);
+        HIP_TRY(c, hipMemcpy(it.data(), c->d_iter, sizeof(int) * c->n_inst, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(act.data(), c->d_active, sizeof(int) * c->n_inst, hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n_inst; ++i) {
+            status[i].iterations = it[i];
+            status[i].stopped = act[i] ? 0 : 1;
+            status[i].last = e3d_monitor{};
+            if (it[i] > 0 && it[i] <= c->hist_cap)
+                HIP_TRY(c, hipMemcpy(&status[i].last, (MonRow*)c->d_hist + (size_t)i * c->hist_cap + it[i] - 1,
+                                     sizeof(MonRow), hipMemcpyDeviceToHost));
+        }
+    }
+    return done;
+}
+
+int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
+    int rc = check_ready(c, -1);
+    if (rc) return rc;
+    if (!c->batch || i < 0 || i >= c->n_inst || !rows || max_rows < 0)
+        return set_error(c, E3D_EINVAL, "e3d_batch_history: bad argument");
+    int it = 0;
+    HIP_TRY(c, hipMemcpy(&it, c->d_iter + i, sizeof(int), hipMemcpyDeviceToHost));
+    const int n = std::min(std::min(it, c->hist_cap), max_rows);
+    if (n > 0)
+        HIP_TRY(c, hipMemcpy(rows, (MonRow*)c->d_hist + (size_t)i * c->hist_cap, sizeof(MonRow) * n,
+                             hipMemcpyDeviceToHost));
+    return n;
+}
+
+int e3d_set_tile_order(e3d_ctx* c, int xcd_contiguous) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    c->tile_map = c->m.tile_map = xcd_contiguous ? 1 : 0;
+    drop_graph(c);
+    return E3D_OK;
 }
 
 // ------------------------------------------------------------- measurement

@@ -43,6 +43,12 @@ struct DMesh {
     const double* sA;     // [nslot][NRp] |S|
     const int* mate;      // [nslot][NRp] face-state record of the other cell of the face; bit 31: this cell is right
     const double* g_r;    // [3][NG] r vector of each ghost on its face (the reflected right vector)
+    // Instances: a context may hold a batch of independent meshes, each with its
+    // own flow conditions; a single mesh is instance 0.
+    const int* blk_inst;  // [nb] instance of each block
+    const double* inst_fs;  // [n_inst][5] freestream primitive state of each instance
+    const int* active;    // [n_inst] nonzero while the instance iterates; null: every instance iterates
+    int tile_map;         // 1: XCD-contiguous tile order (tile = (b & 7) * per + (b >> 3)), 0: tile = workgroup
 };
 
 // Face-state records of the fused path: record k*NRp + c holds the
@@ -70,6 +76,20 @@ struct DState {
     double* specrad;      // [NRp]
     double* part;         // [ntiles][8] per-tile partial sums of the residual monitor
     double* jac;          // [10][NRp] Jacobi values of the residual smoother (allocated on first use)
+};
+
+// Per-instance constants of a batched run's monitors (PostProcessing/src/PostProcessing.cpp:222-250,
+// AerodynamicParameters.cpp:120-132): rotation of the force sums by the inlet angle, stop
+// criterion and iteration limit.
+struct InstParam {
+    double cos_a, sin_a, criterion;
+    int max_iter, pad;
+};
+
+// One monitor row, laid out as e3d_monitor (include/euler3d.h).
+struct MonRow {
+    int iter, stop;
+    double rms[5], drop[5], cl, cd, cmx, cmy, cmz;
 };
 
 }  // namespace e3d

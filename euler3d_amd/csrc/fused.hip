@@ -113,8 +113,8 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
 
 template <int GRAD, bool VENKAT, int MAXS>
 __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
+    const int tile = tile_of(m, ntiles);
+    if (tile >= ntiles || !instance_active(m, tile_instance(m, tile))) return;
     const int c = tile * kWG + threadIdx.x;
     const int nf = m.nf[c];
     if (nf == 0) return;
@@ -211,10 +211,13 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 // time step (Timestep.cpp:67-146), all from the cell's own state at the start
 // of the iteration, with the spectral radius summed in the same face loop.
 template <int FLUX, bool ORDER2, bool DISS, bool LAST, bool FIRST, int MAXS>
-__global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double beta, double alpha, Free fs,
-                                              double cfl, double inv_gm1, int ntiles) {
-    const int tile = xcd_tile(ntiles);
+__global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double beta, double alpha, double cfl,
+                                              double inv_gm1, int ntiles) {
+    const int tile = tile_of(m, ntiles);
     if (tile >= ntiles) return;
+    const int inst = tile_instance(m, tile);
+    if (!instance_active(m, inst)) return;
+    const double* fs = m.inst_fs + 5 * inst;
     const int c = tile * kWG + threadIdx.x;
     const int nf = m.nf[c];
     const int NRp = m.NRp, ns = m.nslot;
@@ -322,7 +325,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
 #pragma unroll
                     for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
                     if ((kind & GK_BC_MASK) == GK_SLIP) ghost_slip(wnew, n, gw);
-                    else ghost_farfield(gas.gamma, wnew, fs.v, n, gw);
+                    else ghost_farfield(gas.gamma, wnew, fs, n, gw);
                 } else {
                     load_w(s.W, m.NA, gcell, gw);
                 }
@@ -365,15 +368,12 @@ void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, 
 }
 
 void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, bool first,
-                        const Gas& g, double beta, double alpha, const double fs[5], double cfl, double inv_gm1,
-                        hipStream_t st) {
+                        const Gas& g, double beta, double alpha, double cfl, double inv_gm1, hipStream_t st) {
     const int nt = m.NRp / kWG;
     const dim3 gr(xcd_grid(nt)), bl(kWG);
-    Free f;
-    for (int v = 0; v < 5; ++v) f.v[v] = fs[v];
     // Stage 0 always has dissipation (beta_0 = 1 in every RK table), so FIRST implies DISS.
 #define FL(FX, O2, DI, LA, FI, S) \
-    hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, FI, S>), gr, bl, 0, st, m, s, g, beta, alpha, f, cfl, inv_gm1, nt)
+    hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, FI, S>), gr, bl, 0, st, m, s, g, beta, alpha, cfl, inv_gm1, nt)
 #define FL_S(FX, O2, DI, LA, FI) { if (m.nslot <= 4) FL(FX, O2, DI, LA, FI, 4); else FL(FX, O2, DI, LA, FI, 6); }
 #define FL_L(FX, O2, DI, FI) { if (last) FL_S(FX, O2, DI, true, FI) else FL_S(FX, O2, DI, false, FI) }
 #define FL_D(FX, O2) { if (first) FL_L(FX, O2, true, true) else if (diss) FL_L(FX, O2, true, false) else FL_L(FX, O2, false, false) }

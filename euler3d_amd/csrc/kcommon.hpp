@@ -16,6 +16,15 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
 }
 inline int xcd_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
 
+// Tile of this workgroup under the mesh's tile order.
+__device__ __forceinline__ int tile_of(const DMesh& m, int ntiles) {
+    return m.tile_map ? xcd_tile(ntiles) : (int)blockIdx.x;
+}
+
+// Instance of a tile, and whether it still iterates.
+__device__ __forceinline__ int tile_instance(const DMesh& m, int tile) { return m.blk_inst[m.wg_block[tile]]; }
+__device__ __forceinline__ bool instance_active(const DMesh& m, int inst) { return !m.active || m.active[inst]; }
+
 struct Free {  // freestream primitive state, by value in kernel arguments
     double v[5];
 };

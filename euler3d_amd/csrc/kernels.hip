@@ -448,8 +448,9 @@ __global__ void __launch_bounds__(kWG) k_rms_partials(DMesh m, DState s, int nti
 __global__ void __launch_bounds__(kWG) k_block_monitors(const int* blk_tile0, const int* blk_ntile, const int* blk_nreal,
                                                         const int* wall_off, const int* wall_ghost, const int* wall_face,
                                                         const double* fcenter, int NF, DMesh m, const double* W,
-                                                        const double* part, double qcmac, double* out) {
+                                                        const double* part, const double* blk_qcmac, double* out) {
     const int b = blockIdx.x;
+    const double qcmac = blk_qcmac[b];  // dynamic head x cmac of the block's instance
     __shared__ double red[kWG / 64][5];
     double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int q = wall_off[b] + threadIdx.x; q < wall_off[b + 1]; q += kWG) {
@@ -564,9 +565,9 @@ void launch_rms_partials(const DMesh& m, const DState& s, hipStream_t st) {
 }
 void launch_block_monitors(int nb, const int* tile0, const int* ntile, const int* nreal, const int* wall_off,
                            const int* wall_ghost, const int* wall_face, const double* fcenter, const DMesh& m,
-                           const double* W, const double* part, double qcmac, double* out, hipStream_t st) {
+                           const double* W, const double* part, const double* blk_qcmac, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_block_monitors, dim3(nb), dim3(kWG), 0, st, tile0, ntile, nreal, wall_off, wall_ghost,
-                       wall_face, fcenter, m.NF, m, W, part, qcmac, out);
+                       wall_face, fcenter, m.NF, m, W, part, blk_qcmac, out);
 }
 
 }  // namespace e3d

@@ -208,6 +208,49 @@ int e3d_reset_monitors(e3d_ctx* ctx);
 int e3d_monitor_state(const e3d_ctx* ctx, int* iter, double rms0[5]);
 int e3d_set_monitor_state(e3d_ctx* ctx, int iter, const double rms0[5]);
 
+/* ------------------------------------------------------------- batches
+ * Many independent meshes ("instances") of one scheme, each with its own flow
+ * conditions, advanced together: every RK stage is one launch over all of them
+ * (no counterpart in the reference, which runs one case per process). An
+ * instance stops on its own stop test (PostProcessing.cpp:59-83, per instance;
+ * a non-finite residual norm also stops it) and its cells are skipped from then
+ * on; its state stays as it was at its last iteration. */
+typedef struct e3d_instance {
+    double mach, aoa_deg;  /* freestream (PrimitiveVariables.cpp:41-56) */
+    double cmac;           /* reference length of the coefficients */
+    double criterion;      /* stop when log10 density residual drop <= criterion */
+    int max_iter;          /* stop at iteration max_iter - 1 */
+} e3d_instance;
+/* Uploads n meshes as one batch. Scheme, gamma, CFL and RK stages come from
+ * e3d_set_params; mach/aoa/cmac/criterion/max_iter of params are not used.
+ * Blocks are numbered across the batch: instance i owns blocks
+ * [first, first + count) of e3d_batch_blocks, and the per-block entry points
+ * (e3d_get_array, e3d_set_array, granular operators) take those numbers.
+ * e3d_init_freestream then sets each instance to its own freestream. */
+int e3d_upload_batch(e3d_ctx* ctx, const e3d_mesh* const* meshes, const e3d_instance* inst, int n);
+int e3d_batch_size(const e3d_ctx* ctx);
+int e3d_batch_blocks(const e3d_ctx* ctx, int inst, int* first_block, int* n_blocks);
+/* Per-instance outcome of e3d_batch_run. */
+typedef struct e3d_batch_status {
+    int iterations;        /* iterations the instance has run */
+    int stopped;           /* 1 once its stop test fired */
+    e3d_monitor last;      /* its last monitor row */
+} e3d_batch_status;
+/* Runs up to max_iterations more iterations, stopping early once every instance
+ * has stopped. use_graph = 1 replays a hipGraph of two iterations (stage
+ * kernels, exchange, per-block monitors, per-instance stop test) with one host
+ * check per 16 iterations; 0 launches the same kernels one by one. status
+ * (may be NULL) receives one entry per instance. Returns the number of
+ * iterations launched or a status < 0. */
+int e3d_batch_run(e3d_ctx* ctx, int max_iterations, int use_graph, e3d_batch_status* status);
+/* Monitor rows of one instance, iterations [0, n) with n = min(max_rows, rows
+ * recorded); returns n. Rows beyond the history capacity (max over the
+ * instances' max_iter, at most 200000) are not kept. */
+int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
+/* Workgroup-to-tile order of the stage kernels: 1 (default) gives each of the
+ * eight XCDs a contiguous range of cell tiles, 0 maps workgroup i to tile i. */
+int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
+
 /* --------------------------------------------------------- measurement */
 /* Average duration in ms of the named kernel over the last e3d_iterate call
  * (HIP events on the context's stream); kernel = "interp", "flux_update", ... */

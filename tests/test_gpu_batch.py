@@ -1,0 +1,137 @@
+"""Batched runs: several meshes with their own flow conditions advanced
+together (e3d_upload_batch / e3d_batch_run), checked instance by instance
+against the same case run alone through e3d_iterate.
+
+The stage kernels are the same, so states and residual norms are equal bit
+for bit; the residual drop is formed on the device in a batch (log10 there)
+and on the host for a single run, so drops are compared to 1e-14."""
+import numpy as np
+import pytest
+
+import euler3d_amd as E
+from helpers import Golden, mesh_for, values_equal
+
+pytestmark = pytest.mark.gpu
+
+SCHEME = dict(flux=E.FLUX_ROE, order=2, gradient=E.GRAD_GREEN_GAUSS, limiter=E.LIM_VENKATAKRISHNAN, k=5.0,
+              stages=5)
+CASES = [  # mesh, mach, aoa
+    (lambda: E.Mesh.bump(8, 3, 3, 0.1, 4, (1, 1, 1)), 0.5, 1.0),
+    (lambda: E.Mesh.bump(12, 4, 4, 0.06, 4, (2, 1, 1)), 0.7, -1.0),
+    (lambda: mesh_for(Golden("naca_hex_roe2_gg_vk")), 0.8, 1.25),
+    (lambda: E.Mesh.bump(10, 3, 3, 0.12, 2, (1, 1, 1)), 0.35, 5.0),
+]
+
+
+def single_run(mesh, mach, aoa, n, criterion=-15.0):
+    ctx = E.Context(0)
+    ctx.set_params(E.make_params(mach, aoa, max_iter=n, criterion=criterion, **SCHEME))
+    ctx.upload(mesh)
+    ctx.init_freestream()
+    hist = ctx.iterate(n)
+    W = [ctx.get(b, E.ARR_W) for b in range(mesh.nblocks)]
+    ctx.close()
+    return W, hist
+
+
+def batch_context(meshes, insts):
+    ctx = E.Context(0)
+    ctx.set_params(E.make_params(0.5, 0.0, **SCHEME))  # mach/aoa of params are not used by a batch
+    ctx.upload_batch(meshes, insts)
+    ctx.init_freestream()
+    return ctx
+
+
+def assert_rows_match(batch_rows, single_rows):
+    assert len(batch_rows) == len(single_rows)
+    for a, r in zip(batch_rows, single_rows):
+        assert a["iter"] == r["iter"] and a["stop"] == r["stop"]
+        assert a["rms"] == r["rms"], (a["iter"], a["rms"], r["rms"])
+        assert [a["cl"], a["cd"], a["cmx"], a["cmy"], a["cmz"]] == [r["cl"], r["cd"], r["cmx"], r["cmy"], r["cmz"]]
+        assert np.allclose(a["drop"], r["drop"], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_instances_match_single_runs(use_graph):
+    meshes = [mk() for mk, _, _ in CASES]
+    iters = [12, 7, 10, 12]  # uneven stops: each instance stops at its own max_iter
+    insts = [dict(mach=m, aoa_deg=a, cmac=1.0, criterion=-15.0, max_iter=n) for (_, m, a), n in zip(CASES, iters)]
+    ctx = batch_context(meshes, insts)
+    print(f"batch of {len(meshes)} instances, graph={use_graph}", flush=True)
+    done, status = ctx.batch_run(40, use_graph=use_graph)
+    assert done >= max(iters)
+    for i, ((mk, mach, aoa), n) in enumerate(zip(CASES, iters)):
+        assert status[i]["stopped"] == 1 and status[i]["iterations"] == n
+        W, hist = single_run(mk(), mach, aoa, n)
+        blocks = ctx.batch_blocks(i)
+        assert len(blocks) == len(W)
+        for b, Wb in zip(blocks, W):
+            mine = ctx.get(b, E.ARR_W)
+            assert values_equal(mine, Wb).all(), f"instance {i} block {b}"
+            assert (mine.view(np.int64) == Wb.view(np.int64)).all(), f"instance {i} block {b}: sign of zero"
+        assert_rows_match(ctx.batch_history(i, 100), hist)
+        assert status[i]["last"]["iter"] == n - 1 and status[i]["last"]["stop"] == 1
+    ctx.close()
+
+
+def test_batch_graph_equals_eager_and_resumes():
+    """Graph replays and one-by-one launches give the same bits, also when a
+    run is split into several calls of odd lengths."""
+    meshes_a = [mk() for mk, _, _ in CASES[:2]]
+    meshes_b = [mk() for mk, _, _ in CASES[:2]]
+    insts = [dict(mach=m, aoa_deg=a, criterion=-15.0, max_iter=30) for _, m, a in CASES[:2]]
+    a = batch_context(meshes_a, insts)
+    b = batch_context(meshes_b, insts)
+    a.batch_run(11, use_graph=False)
+    for n in (3, 5, 3):
+        b.batch_run(n, use_graph=True)
+    for blk in range(len(a.sizes)):
+        assert (a.get(blk, E.ARR_W).view(np.int64) == b.get(blk, E.ARR_W).view(np.int64)).all()
+    for i in range(2):
+        assert a.batch_history(i, 50) == b.batch_history(i, 50)
+    a.close()
+    b.close()
+
+
+def test_batch_criterion_stops_instance_and_freezes_it():
+    """An instance whose criterion is met stops; its state is left as it was at
+    its last iteration while the others go on."""
+    meshes = [mk() for mk, _, _ in CASES[:2]]
+    insts = [dict(mach=CASES[0][1], aoa_deg=CASES[0][2], criterion=-0.5, max_iter=400),
+             dict(mach=CASES[1][1], aoa_deg=CASES[1][2], criterion=-15.0, max_iter=60)]
+    ctx = batch_context(meshes, insts)
+    done, status = ctx.batch_run(200, use_graph=True)
+    st0 = status[0]
+    assert st0["stopped"] == 1 and st0["last"]["drop"][0] <= -0.5 and st0["iterations"] < 60
+    assert status[1]["stopped"] == 1 and status[1]["iterations"] == 60
+    rows = ctx.batch_history(0, 1000)
+    assert len(rows) == st0["iterations"] and all(r["drop"][0] > -0.5 for r in rows[:-1])
+    W, _ = single_run(CASES[0][0](), CASES[0][1], CASES[0][2], st0["iterations"])
+    assert (ctx.get(0, E.ARR_W).view(np.int64) == W[0].view(np.int64)).all()
+    ctx.close()
+
+
+def test_batch_tile_order_does_not_change_results():
+    meshes = [mk() for mk, _, _ in CASES[:2]]
+    insts = [dict(mach=m, aoa_deg=a, criterion=-15.0, max_iter=20) for _, m, a in CASES[:2]]
+    a = batch_context(meshes, insts)
+    b = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
+    b.set_tile_order(False)
+    a.batch_run(8, use_graph=True)
+    b.batch_run(8, use_graph=True)
+    for blk in range(len(a.sizes)):
+        assert (a.get(blk, E.ARR_W).view(np.int64) == b.get(blk, E.ARR_W).view(np.int64)).all()
+    assert a.batch_history(1, 20) == b.batch_history(1, 20)
+    a.close()
+    b.close()
+
+
+def test_batch_api_errors():
+    ctx = batch_context([CASES[0][0]()], [dict(mach=0.5, max_iter=5)])
+    with pytest.raises(E.E3DError):
+        ctx.iterate(1)               # a batch advances with batch_run
+    with pytest.raises(E.E3DError):
+        ctx.monitors()
+    with pytest.raises(E.E3DError):
+        ctx.batch_history(3, 10)     # no such instance
+    ctx.close()
