@@ -67,7 +67,7 @@ class Instance(C.Structure):
 
 
 class BatchStatus(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("stopped", C.c_int), ("last", Monitor)]
+    _fields_ = [("iterations", C.c_int), ("stopped", C.c_int), ("last", Monitor), ("stop_seconds", C.c_double)]
 
 
 class E3DError(RuntimeError):
@@ -289,7 +289,8 @@ class Context:
         n = lib().e3d_batch_size(self._h)
         st = (BatchStatus * n)()
         done = self._c(lib().e3d_batch_run(self._h, max_iterations, 1 if use_graph else 0, st))
-        return done, [{"iterations": x.iterations, "stopped": x.stopped, "last": x.last.as_dict()} for x in st]
+        return done, [{"iterations": x.iterations, "stopped": x.stopped, "last": x.last.as_dict(),
+                       "stop_seconds": x.stop_seconds} for x in st]
 
     def batch_history(self, i, max_rows):
         """Monitor rows of instance i, at most max_rows (from iteration 0)."""

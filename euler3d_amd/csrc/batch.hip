@@ -21,7 +21,7 @@ namespace e3d {
 // a non-finite norm also stops), the history row, and the active/frozen flags.
 __global__ void k_batch_reduce(int n_inst, const int* inst_b0, const double* blk_rows, const InstParam* prm,
                                int* iter, double* rms0, int* active, int* frozen, MonRow* hist, int hist_cap,
-                               int* n_active) {
+                               int* n_active, long long* stop_clock) {
     __shared__ int cnt[kWG];
     int mine = 0;
     for (int i = threadIdx.x; i < n_inst; i += blockDim.x) {
@@ -62,6 +62,7 @@ __global__ void k_batch_reduce(int n_inst, const int* inst_b0, const double* blk
         if (stop) {
             active[i] = 0;
             frozen[i] = 1;
+            stop_clock[i] = wall_clock64();
         } else {
             ++mine;
         }
@@ -87,12 +88,17 @@ __global__ void __launch_bounds__(kWG) k_batch_freeze(DMesh m, const int* frozen
     for (int v = 0; v < 5; ++v) Wn[(size_t)v * m.NA + x] = W[(size_t)v * m.NA + x];
 }
 
+// Device wall clock (constant rate) at the start of a run.
+__global__ void k_clock_mark(long long* t) { *t = wall_clock64(); }
+
 void launch_batch_reduce(int n_inst, const int* inst_b0, const double* blk_rows, const void* prm, int* iter,
                          double* rms0, int* active, int* frozen, void* hist, int hist_cap, int* n_active,
-                         hipStream_t st) {
+                         long long* stop_clock, hipStream_t st) {
     hipLaunchKernelGGL(k_batch_reduce, dim3(1), dim3(kWG), 0, st, n_inst, inst_b0, blk_rows, (const InstParam*)prm,
-                       iter, rms0, active, frozen, (MonRow*)hist, hist_cap, n_active);
+                       iter, rms0, active, frozen, (MonRow*)hist, hist_cap, n_active, stop_clock);
 }
+
+void launch_clock_mark(long long* t, hipStream_t st) { hipLaunchKernelGGL(k_clock_mark, dim3(1), dim3(1), 0, st, t); }
 
 void launch_batch_freeze(const DMesh& m, const int* frozen, const int* g_blk, const double* W, double* Wn,
                          hipStream_t st) {

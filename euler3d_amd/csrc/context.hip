@@ -33,7 +33,8 @@ void launch_block_monitors(int, const int*, const int*, const int*, const int*, 
                            hipStream_t);
 void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
 void launch_batch_reduce(int, const int*, const double*, const void*, int*, double*, int*, int*, void*, int, int*,
-                         hipStream_t);
+                         long long*, hipStream_t);
+void launch_clock_mark(long long*, hipStream_t);
 void launch_batch_freeze(const DMesh&, const int*, const int*, const double*, double*, hipStream_t);
 
 void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, bool, const Gas&, double, double, double,
@@ -91,6 +92,8 @@ struct e3d_ctx {
     void* d_hist = nullptr;
     int hist_cap = 0;
     int* h_nactive = nullptr;  // pinned
+    long long* d_clock = nullptr;  // [0] device wall clock at the start of the run, [1 + i] when instance i stopped
+    bool clock_started = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double* graph_W = nullptr;   // W buffer the graph was captured with
@@ -568,7 +571,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         (rc = dalloc(c, &c->d_inst_fs, 5 * (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_blk_qcmac, (size_t)nb)) ||
         (rc = dalloc(c, &c->d_active, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_frozen, (size_t)c->n_inst)) ||
         (rc = dalloc(c, &c->d_iter, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_nactive, 1)) ||
-        (rc = dalloc(c, &c->d_rms0, 5 * (size_t)c->n_inst)) ||
+        (rc = dalloc(c, &c->d_rms0, 5 * (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_clock, 1 + (size_t)c->n_inst)) ||
         (rc = dalloc(c, (char**)&c->d_prm, sizeof(InstParam) * (size_t)c->n_inst))) {
         free_device(c);
         return rc;
@@ -632,6 +635,7 @@ int e3d_init_freestream(e3d_ctx* c) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->iter = 0;
     c->started = false;
+    c->clock_started = false;
     return E3D_OK;
 }
 
@@ -997,7 +1001,7 @@ static int batch_iteration(e3d_ctx* c) {
     launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
                           c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mon, c->stream);
     launch_batch_reduce(c->n_inst, c->d_inst_b0, c->d_mon, c->d_prm, c->d_iter, c->d_rms0, c->d_active, c->d_frozen,
-                        c->d_hist, c->hist_cap, c->d_nactive, c->stream);
+                        c->d_hist, c->hist_cap, c->d_nactive, c->d_clock + 1, c->stream);
     launch_batch_freeze(c->m, c->d_frozen, c->d_g_blk, c->s.W, c->s.Wn, c->stream);
     ev_end(c, ev);
     return launch_status(c);
@@ -1044,6 +1048,10 @@ int e3d_batch_run(e3d_ctx* c, int max_it, int use_graph, e3d_batch_status* statu
         if ((rc = exchange(c, 0))) return rc;
         c->started = true;
     }
+    if (!c->clock_started) {
+        launch_clock_mark(c->d_clock, c->stream);
+        c->clock_started = true;
+    }
     int done = 0, n_active = 1;
     if (use_graph) {
         constexpr int kReplays = 8;  // graph launches (two iterations each) between host checks
@@ -1063,11 +1071,16 @@ int e3d_batch_run(e3d_ctx* c, int max_it, int use_graph, e3d_batch_status* statu
     }
     if (status) {
         std::vector<int> it(c->n_inst), act(c->n_inst);
+        std::vector<long long> clk(c->n_inst + 1);
         HIP_TRY(c, hipMemcpy(it.data(), c->d_iter, sizeof(int) * c->n_inst, hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemcpy(act.data(), c->d_active, sizeof(int) * c->n_inst, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(clk.data(), c->d_clock, sizeof(long long) * clk.size(), hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIP_TRY(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
         for (int i = 0; i < c->n_inst; ++i) {
             status[i].iterations = it[i];
             status[i].stopped = act[i] ? 0 : 1;
+            status[i].stop_seconds = act[i] || khz <= 0 ? 0.0 : (double)(clk[1 + i] - clk[0]) / (1e3 * khz);
             status[i].last = e3d_monitor{};
             if (it[i] > 0 && it[i] <= c->hist_cap)
                 HIP_TRY(c, hipMemcpy(&status[i].last, (MonRow*)c->d_hist + (size_t)i * c->hist_cap + it[i] - 1,

@@ -235,6 +235,8 @@ typedef struct e3d_batch_status {
     int iterations;        /* iterations the instance has run */
     int stopped;           /* 1 once its stop test fired */
     e3d_monitor last;      /* its last monitor row */
+    double stop_seconds;   /* device time from the first e3d_batch_run after e3d_init_freestream
+                              to the end of its last iteration (0 while running) */
 } e3d_batch_status;
 /* Runs up to max_iterations more iterations, stopping early once every instance
  * has stopped. use_graph = 1 replays a hipGraph of two iterations (stage
