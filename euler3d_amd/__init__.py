@@ -23,6 +23,7 @@ BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 
 class BumpDesc(C.Structure):
@@ -63,7 +64,7 @@ class Monitor(C.Structure):
 
 class Instance(C.Structure):
     _fields_ = [("mach", C.c_double), ("aoa_deg", C.c_double), ("cmac", C.c_double), ("criterion", C.c_double),
-                ("max_iter", C.c_int)]
+                ("max_iter", C.c_int), ("rms_ref", C.c_double * 5)]
 
 
 class BatchStatus(C.Structure):
@@ -127,6 +128,17 @@ def lib():
             "e3d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(BatchStatus)]),
             "e3d_batch_history": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor), C.c_int]),
             "e3d_set_tile_order": (C.c_int, [C.c_void_p, C.c_int]),
+            "e3d_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                                         C.POINTER(C.c_void_p)]),
+            "e3d_mlp_destroy": (None, [C.c_void_p]),
+            "e3d_mlp_param_count": (C.c_long, [C.c_void_p]),
+            "e3d_mlp_get_params": (C.c_int, [C.c_void_p, _fp]),
+            "e3d_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+            "e3d_mlp_set_data": (C.c_int, [C.c_void_p, _fp, _fp, C.c_long]),
+            "e3d_mlp_gradients": (C.c_int, [C.c_void_p, _ip, C.c_int, _fp, _fp]),
+            "e3d_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_ulonglong, _fp]),
+            "e3d_mlp_predict": (C.c_int, [C.c_void_p, _fp, C.c_long, _fp]),
+            "e3d_mlp_time_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
         }
         for name in ("e3d_save_w0", "e3d_spectral_radius", "e3d_timestep", "e3d_interpolation",
                      "e3d_smooth_residual", "e3d_bc", "e3d_runge_kutta"):
@@ -270,7 +282,8 @@ class Context:
         n = len(meshes)
         hs = (C.c_void_p * n)(*[m._h.value for m in meshes])
         ins = (Instance * n)(*[Instance(d["mach"], d.get("aoa_deg", 0.0), d.get("cmac", 1.0),
-                                        d.get("criterion", -8.0), d.get("max_iter", 1000)) for d in instances])
+                                        d.get("criterion", -8.0), d.get("max_iter", 1000),
+                                        (C.c_double * 5)(*d.get("rms_ref", [0.0] * 5))) for d in instances])
         self._c(lib().e3d_upload_batch(self._h, hs, ins, n))
         self.sizes = []
         for mesh in meshes:
@@ -341,6 +354,69 @@ class Context:
     def close(self):
         if self._h and self._h.value:
             lib().e3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MLP:
+    """Coordinate MLP of the learned warm start (BF16 matrix-core kernels, FP32 master weights)."""
+
+    def __init__(self, n_in, n_out, width=256, depth=5, seed=0, device=0):
+        out = C.c_void_p()
+        _check(lib().e3d_mlp_create(device, n_in, width, depth, n_out, seed, C.byref(out)))
+        self._h = C.c_void_p(out.value)
+        self.n_in, self.n_out, self.width, self.depth = n_in, n_out, width, depth
+
+    def param_count(self):
+        return int(lib().e3d_mlp_param_count(self._h))
+
+    def get_params(self):
+        out = np.empty(self.param_count(), np.float32)
+        _check(lib().e3d_mlp_get_params(self._h, out.ctypes.data_as(_fp)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().e3d_mlp_set_params(self._h, p.ctypes.data_as(_fp)))
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        assert x.shape[1] == self.n_in and t.shape[1] == self.n_out and len(x) == len(t)
+        self._x, self._t = x, t
+        _check(lib().e3d_mlp_set_data(self._h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), len(x)))
+
+    def gradients(self, idx):
+        idx = np.ascontiguousarray(idx, np.int32)
+        g = np.empty(self.param_count(), np.float32)
+        loss = C.c_float()
+        _check(lib().e3d_mlp_gradients(self._h, idx.ctypes.data_as(_ip), len(idx), C.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def train(self, steps, batch=8192, lr=1e-3, cosine=True, seed=0):
+        loss = np.empty(max(steps, 1), np.float32)
+        _check(lib().e3d_mlp_train(self._h, steps, batch, lr, 1 if cosine else 0, seed, loss.ctypes.data_as(_fp)))
+        return loss[:steps]
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((len(x), self.n_out), np.float32)
+        _check(lib().e3d_mlp_predict(self._h, x.ctypes.data_as(_fp), len(x), y.ctypes.data_as(_fp)))
+        return y
+
+    def time_step(self, batch, reps=10):
+        ms, fl = C.c_double(), C.c_double()
+        _check(lib().e3d_mlp_time_step(self._h, batch, reps, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().e3d_mlp_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

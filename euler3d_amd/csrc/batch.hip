@@ -40,7 +40,7 @@ __global__ void k_batch_reduce(int n_inst, const int* inst_b0, const double* blk
             sum[9] += r[9];
         }
         const int it = iter[i];
-        if (it == 0)
+        if (it == 0 && !p.has_ref)
             for (int v = 0; v < 5; ++v) rms0[5 * i + v] = sum[v];
         MonRow row;
         row.iter = it;

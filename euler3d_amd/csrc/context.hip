@@ -296,14 +296,14 @@ void freestream_state(double gamma, double mach, double aoa_deg, double fs[5]) {
 int sync_instances(e3d_ctx* c) {
     if (!c->have_params || !c->have_mesh) return E3D_OK;
     const e3d_params& p = c->params;
-    if (!c->batch) c->inst.assign(1, e3d_instance{p.mach, p.aoa_deg, p.cmac, p.criterion, p.max_iter});
+    if (!c->batch) c->inst.assign(1, e3d_instance{p.mach, p.aoa_deg, p.cmac, p.criterion, p.max_iter, {0, 0, 0, 0, 0}});
     c->inst_fs.assign(5 * (size_t)c->n_inst, 0.0);
     std::vector<InstParam> prm(c->n_inst);
     for (int i = 0; i < c->n_inst; ++i) {
         const e3d_instance& q = c->inst[i];
         freestream_state(p.gamma, q.mach, q.aoa_deg, &c->inst_fs[5 * (size_t)i]);
         const double aoa_rad = q.aoa_deg * 4. * std::atan(1.) / 180;  // as the monitor sums of finish_monitor
-        prm[i] = InstParam{std::cos(aoa_rad), std::sin(aoa_rad), q.criterion, q.max_iter, 0};
+        prm[i] = InstParam{std::cos(aoa_rad), std::sin(aoa_rad), q.criterion, q.max_iter, q.rms_ref[0] > 0.0 ? 1 : 0};
     }
     for (int v = 0; v < 5; ++v) c->fs[v] = c->inst_fs[v];
     std::vector<double> qcmac(c->nb);
@@ -632,6 +632,10 @@ int e3d_init_freestream(e3d_ctx* c) {
     HIP_TRY(c, hipMemcpyAsync(c->d_active, ones.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_frozen, zeros.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_iter, zeros.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> ref(5 * (size_t)c->n_inst);
+    for (int i = 0; i < c->n_inst; ++i)
+        for (int v = 0; v < 5; ++v) ref[5 * i + v] = c->inst[i].rms_ref[v];
+    HIP_TRY(c, hipMemcpyAsync(c->d_rms0, ref.data(), sizeof(double) * ref.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->iter = 0;
     c->started = false;

@@ -83,7 +83,7 @@ struct DState {
 // criterion and iteration limit.
 struct InstParam {
     double cos_a, sin_a, criterion;
-    int max_iter, pad;
+    int max_iter, has_ref;  // has_ref: the drop's reference norms were given, not taken at iteration 0
 };
 
 // One monitor row, laid out as e3d_monitor (include/euler3d.h).

@@ -220,6 +220,9 @@ typedef struct e3d_instance {
     double cmac;           /* reference length of the coefficients */
     double criterion;      /* stop when log10 density residual drop <= criterion */
     int max_iter;          /* stop at iteration max_iter - 1 */
+    double rms_ref[5];     /* residual norms the drop is measured from; all 0: the instance's own
+                              iteration-0 norms (PostProcessing.cpp:240-250). A warm start passes the
+                              cold start's iteration-0 norms here. */
 } e3d_instance;
 /* Uploads n meshes as one batch. Scheme, gamma, CFL and RK stages come from
  * e3d_set_params; mach/aoa/cmac/criterion/max_iter of params are not used.
@@ -252,6 +255,38 @@ int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
 /* Workgroup-to-tile order of the stage kernels: 1 (default) gives each of the
  * eight XCDs a contiguous range of cell tiles, 0 maps workgroup i to tile i. */
 int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
+
+/* ------------------------------------------------------ learned warm start
+ * A coordinate MLP that predicts an initial flow field, replacing the uniform
+ * freestream of PrimitiveVariables::initializeFlowField
+ * (DataStructure/src/PrimitiveVariables.cpp:58-69; no counterpart in the
+ * reference otherwise). n_in features (<= 32) -> depth hidden layers of width
+ * units (a multiple of 128) with SiLU -> n_out outputs (<= 32). Trained and
+ * evaluated on the GPU with BF16 operands and FP32 accumulation on the matrix
+ * cores; master weights and Adam state FP32. The solver's own residual decides
+ * convergence: a prediction only replaces the starting state. */
+typedef struct e3d_mlp e3d_mlp;
+int e3d_mlp_create(int device, int n_in, int width, int depth, int n_out, unsigned long long seed, e3d_mlp** out);
+void e3d_mlp_destroy(e3d_mlp* mlp);
+/* Number of parameters; flat layout per layer: W [out x in] row-major, then b [out]. */
+long e3d_mlp_param_count(const e3d_mlp* mlp);
+int e3d_mlp_get_params(e3d_mlp* mlp, float* out);
+int e3d_mlp_set_params(e3d_mlp* mlp, const float* in);
+/* Training set in device memory: x [n x n_in], targets t [n x n_out] (host arrays). */
+int e3d_mlp_set_data(e3d_mlp* mlp, const float* x, const float* t, long n);
+/* Mean-squared-error loss and its gradients (flat, as the parameters) on the
+ * samples idx[0..batch) of the training set, without an update; batch a
+ * multiple of 512. */
+int e3d_mlp_gradients(e3d_mlp* mlp, const int* idx, int batch, float* loss, float* grads);
+/* steps Adam steps on mini-batches (multiple of 512) drawn without replacement
+ * from a seeded shuffle of the training set; cosine != 0 decays lr to lr/20
+ * over the steps; loss (may be NULL) receives one value per step. */
+int e3d_mlp_train(e3d_mlp* mlp, int steps, int batch, float lr, int cosine, unsigned long long seed, float* loss);
+/* Outputs y [n x n_out] for features x [n x n_in] (host arrays). */
+int e3d_mlp_predict(e3d_mlp* mlp, const float* x, long n, float* y);
+/* Average time of one training step's forward and backward passes on a batch
+ * (HIP events, reps repetitions) and its matrix-core flops. */
+int e3d_mlp_time_step(e3d_mlp* mlp, int batch, int reps, double* ms, double* flops);
 
 /* --------------------------------------------------------- measurement */
 /* Average duration in ms of the named kernel over the last e3d_iterate call
