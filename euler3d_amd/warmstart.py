@@ -1,0 +1,257 @@
+"""Learned warm start of the bump-channel solver.
+
+A coordinate MLP (``euler3d_amd.MLP``: BF16 matrix-core kernels, trained on the
+GPU) maps (Mach, inlet angle, bump height, cell centroid, distance to the
+wall) to the converged primitive state, as a deviation from the freestream.
+Its prediction replaces the uniform freestream start
+(DataStructure/src/PrimitiveVariables.cpp:58-69); the solver then converges
+the field with its own method and stop test, so the answer stays the
+method's. The drop of a warm run is measured from the cold start's
+iteration-0 residual of the same case (``rms_ref`` of a batch instance).
+
+Pipeline (``python -m euler3d_amd.warmstart``):
+1. training fields: ``--train`` conditions (seeds 0..n-1) solved from
+   freestream by the batched solver to the stop criterion or ``--max-iter``;
+2. training: ``--steps`` Adam steps on all cells of all training fields;
+3. evaluation: ``--heldout`` conditions (seeds 5000..) run cold and warm in
+   two batches of the same composition; per instance the iterations and the
+   device time to the criterion, with the surrogate's inference time added to
+   the warm run, and the converged Cl / Cd of both.
+Conditions: Mach ~ U[0.3, 0.8], inlet angle ~ U[-2, 6] deg, h ~ U[0.04, 0.12].
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import (ARR_W, FLUX_AUSM, FLUX_ROE, GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES, LIM_BARTH_JESPERSEN,
+               LIM_VENKATAKRISHNAN, MLP, Context, Mesh, make_params)
+
+GAMMA = 1.4
+N_FEATURES = 25   # 3 flow parameters, 3 coordinates, wall distance and its log, 17 Fourier terms
+TARGET_SCALE = 0.2
+
+
+def conditions(seeds):
+    out = []
+    for s in seeds:
+        rng = np.random.default_rng(s)
+        out.append(dict(seed=int(s), mach=float(rng.uniform(0.3, 0.8)), aoa_deg=float(rng.uniform(-2.0, 6.0)),
+                        h=float(rng.uniform(0.04, 0.12))))
+    return out
+
+
+def wall_y(x, h):
+    """Height of the lower wall: the circular-arc bump of thickness h over x in [1, 2]."""
+    R = (0.25 + h * h) / (2.0 * h)
+    y = np.sqrt(np.maximum(R * R - (x - 1.5) ** 2, 0.0)) - (R - h)
+    return np.where((x >= 1.0) & (x <= 2.0), np.maximum(y, 0.0), 0.0)
+
+
+def freestream(mach, aoa_deg):
+    a = math.radians(aoa_deg)
+    return np.array([1.0, mach * math.sqrt(GAMMA) * math.cos(a), mach * math.sqrt(GAMMA) * math.sin(a), 0.0, 1.0])
+
+
+def features(block, cond):
+    """Network inputs of every cell of a block (real cells, then ghosts), about [-1, 1]: flow
+    parameters, centroid, distance above the wall (linear and logarithmic), and sines and
+    cosines of the streamwise coordinate (4 octaves), the wall-normal one (3) and the spanwise
+    one (1), so that the network can place the sharp features near the bump."""
+    c = block["ccenter"].reshape(-1, 3)
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    d = y - wall_y(x, cond["h"])
+    f = np.empty((len(c), N_FEATURES), np.float32)
+    f[:, 0] = (cond["mach"] - 0.55) / 0.25
+    f[:, 1] = (cond["aoa_deg"] - 2.0) / 4.0
+    f[:, 2] = (cond["h"] - 0.08) / 0.04
+    f[:, 3] = x / 1.5 - 1.0
+    f[:, 4] = 2.0 * y - 1.0
+    f[:, 5] = 2.0 * z - 1.0
+    f[:, 6] = 2.0 * d
+    f[:, 7] = np.log10(np.abs(d) + 1e-3) / 3.0 + 0.5
+    k = 8
+    for coord, octaves in ((x / 3.0, 4), (d, 3), (z, 1)):
+        for j in range(octaves):
+            w = np.pi * 2.0 ** j
+            f[:, k] = np.sin(w * coord)
+            f[:, k + 1] = np.cos(w * coord)
+            k += 2
+    f[:, k] = np.sin(np.pi * 2.0 * x / 3.0) * np.exp(-8.0 * d)  # streamwise shape of the near-wall layer
+    assert k + 1 == N_FEATURES
+    return f
+
+
+def targets(W, cond, nreal):
+    """Training targets of the real cells: (W - freestream) / scale, [n x 5]."""
+    fs = freestream(cond["mach"], cond["aoa_deg"])
+    return ((W[:, :nreal].T - fs) / TARGET_SCALE).astype(np.float32)
+
+
+def state_from_prediction(y, cond):
+    """Primitive state [5 x n] from network outputs [n x 5]."""
+    return (y.astype(np.float64) * TARGET_SCALE + freestream(cond["mach"], cond["aoa_deg"])).T
+
+
+def scheme_params(order, flux="roe", gradient="gg", limiter="vk", k=5.0, omega=1e-16):
+    return dict(flux=FLUX_AUSM if flux == "ausm" else FLUX_ROE, order=order,
+                gradient=GRAD_LEAST_SQUARES if gradient == "lsq" else GRAD_GREEN_GAUSS,
+                limiter=LIM_BARTH_JESPERSEN if limiter == "bj" else LIM_VENKATAKRISHNAN, k=k, omega=omega, stages=5)
+
+
+def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=None, use_graph=True):
+    """Runs one batch; warm: per instance [5 x nall] initial states (None: freestream).
+    Returns per-instance results and the final real-cell states."""
+    ctx = Context(0)
+    ctx.set_params(make_params(0.5, 0.0, max_iter=max_iter, criterion=criterion, **scheme))
+    insts = []
+    for i, c in enumerate(conds):
+        d = dict(mach=c["mach"], aoa_deg=c["aoa_deg"], cmac=1.0, criterion=criterion, max_iter=max_iter)
+        if rms_ref is not None:
+            d["rms_ref"] = list(rms_ref[i])
+        insts.append(d)
+    ctx.upload_batch(meshes, insts)
+    ctx.init_freestream()
+    if warm is not None:
+        for i, Wi in enumerate(warm):
+            (b,) = ctx.batch_blocks(i)
+            ctx.set(b, ARR_W, Wi)
+    ctx.op("sync")
+    t0 = time.perf_counter()
+    done, status = ctx.batch_run(max_iter, use_graph=use_graph)
+    wall = time.perf_counter() - t0
+    out = []
+    states = []
+    for i, (c, s) in enumerate(zip(conds, status)):
+        (b,) = ctx.batch_blocks(i)
+        first = ctx.batch_history(i, 1)
+        out.append(dict(c, iterations=s["iterations"], stop_seconds=s["stop_seconds"], drop=s["last"]["drop"][0],
+                        cl=s["last"]["cl"], cd=s["last"]["cd"], rms0=first[0]["rms"] if first else None,
+                        drop0=first[0]["drop"][0] if first else None,
+                        converged=bool(s["last"]["drop"][0] <= criterion)))
+        states.append(ctx.get(b, ARR_W))
+    ctx.close()
+    return out, states, wall
+
+
+def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000, criterion=-8.0, steps=3000,
+        batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, log=sys.stderr):
+    scheme = scheme_params(order)
+    train_c = conditions(range(n_train))
+    held_c = conditions(range(5000, 5000 + n_heldout))
+
+    def meshes_for(conds):
+        return [Mesh.bump(nx, ny, nz, c["h"], 4, (1, 1, 1)) for c in conds]
+
+    # 1. training fields from the batched solver
+    t0 = time.perf_counter()
+    tm = meshes_for(train_c)
+    res, states, t_gen = solve_batch(tm, train_c, scheme, max_iter, criterion)
+    X, T = [], []
+    for m, c, W in zip(tm, train_c, states):
+        blk = m.block(0)
+        if not np.isfinite(W).all():
+            continue  # a diverged case is no training data
+        X.append(features(blk, c)[:blk["nreal"]])
+        T.append(targets(W, c, blk["nreal"]))
+    X, T = np.concatenate(X), np.concatenate(T)
+    print(f"[warmstart] {len(train_c)} training fields ({sum(r['converged'] for r in res)} converged) in "
+          f"{t_gen:.1f}s; {len(X)} samples", file=log, flush=True)
+    # 2. training on the GPU
+    net = MLP(N_FEATURES, 5, width, depth, seed=1)
+    net.set_data(X, T)
+    t1 = time.perf_counter()
+    loss = net.train(steps, batch=batch, lr=lr, seed=2)
+    t_train = time.perf_counter() - t1
+    ms, flops = net.time_step(batch, reps=5)
+    print(f"[warmstart] trained {steps} steps in {t_train:.1f}s, loss {loss[0]:.3e} -> {np.mean(loss[-50:]):.3e}; "
+          f"step {ms:.3f} ms = {flops / ms / 1e9:.1f} TFLOP/s", file=log, flush=True)
+    # 3. held-out: cold, then warm from the network with the cold iteration-0 norms as reference
+    hm = meshes_for(held_c)
+    cold, cold_states, t_cold = solve_batch(hm, held_c, scheme, max_iter, criterion)
+    t2 = time.perf_counter()
+    warm_states = []
+    for m, c in zip(hm, held_c):
+        blk = m.block(0)
+        warm_states.append(state_from_prediction(net.predict(features(blk, c)), c))
+    t_infer = time.perf_counter() - t2
+    warm, _, t_warm = solve_batch(hm, held_c, scheme, max_iter, criterion, warm=warm_states,
+                                  rms_ref=[r["rms0"] for r in cold])
+    # converged coefficients: both starts taken `deeper` decades further, where Cl and Cd no longer move
+    deep_cold = deep_warm = None
+    if deeper > 0:
+        deep_cold, _, _ = solve_batch(hm, held_c, scheme, 2 * max_iter, criterion - deeper)
+        deep_warm, _, _ = solve_batch(hm, held_c, scheme, 2 * max_iter, criterion - deeper, warm=warm_states,
+                                      rms_ref=[r["rms0"] for r in cold])
+    rows = []
+    for c, w, cs, ws, m in zip(cold, warm, cold_states, warm_states, hm):
+        nr = m.block(0)["nreal"]
+        field_err = float(np.max(np.abs(ws[:, :nr] - cs[:, :nr])))  # surrogate vs converged cold state, real cells
+        rows.append(dict(seed=c["seed"], mach=c["mach"], aoa_deg=c["aoa_deg"], h=c["h"], field_max_err=field_err,
+                         cold_iterations=c["iterations"], warm_iterations=w["iterations"],
+                         cold_seconds=c["stop_seconds"], warm_seconds=w["stop_seconds"] + t_infer / len(held_c),
+                         cold_converged=c["converged"], warm_converged=w["converged"],
+                         warm_initial_drop=w["drop0"], cl_cold=c["cl"], cl_warm=w["cl"], cd_cold=c["cd"], cd_warm=w["cd"],
+                         cl_rel_diff=abs(w["cl"] - c["cl"]) / max(abs(c["cl"]), 1e-300),
+                         cd_rel_diff=abs(w["cd"] - c["cd"]) / max(abs(c["cd"]), 1e-300)))
+    if deeper > 0:
+        for r, dc, dw in zip(rows, deep_cold, deep_warm):
+            r.update(deep_converged=bool(dc["converged"] and dw["converged"]), cl_deep_cold=dc["cl"],
+                     cl_deep_warm=dw["cl"], cd_deep_cold=dc["cd"], cd_deep_warm=dw["cd"],
+                     cl_deep_rel_diff=abs(dw["cl"] - dc["cl"]) / max(abs(dc["cl"]), 1e-300),
+                     cd_deep_rel_diff=abs(dw["cd"] - dc["cd"]) / max(abs(dc["cd"]), 1e-300))
+    both = [r for r in rows if r["cold_converged"] and r["warm_converged"]]
+    summary = dict(n_train=n_train, n_heldout=n_heldout, mesh=[nx, ny, nz], order=order, criterion=criterion,
+                   max_iter=max_iter, samples=int(len(X)), train_steps=steps, batch=batch,
+                   train_seconds=t_train, train_step_ms=ms, train_tflops=flops / ms / 1e9,
+                   loss_first=float(loss[0]), loss_last=float(np.mean(loss[-50:])), generate_seconds=t_gen,
+                   inference_seconds=t_infer, cold_batch_seconds=t_cold, warm_batch_seconds=t_warm,
+                   converged_both=len(both),
+                   iterations_ratio=(sum(r["cold_iterations"] for r in both) / max(1, sum(r["warm_iterations"] for r in both))),
+                   time_ratio=(sum(r["cold_seconds"] for r in both) / max(1e-12, sum(r["warm_seconds"] for r in both))),
+                   max_cl_rel_diff=max([r["cl_rel_diff"] for r in both], default=None),
+                   max_cd_rel_diff=max([r["cd_rel_diff"] for r in both], default=None),
+                   deeper=deeper,
+                   max_cl_deep_rel_diff=max([r["cl_deep_rel_diff"] for r in rows if r.get("deep_converged")],
+                                            default=None),
+                   max_cd_deep_rel_diff=max([r["cd_deep_rel_diff"] for r in rows if r.get("deep_converged")],
+                                            default=None),
+                   heldout=rows, wall_seconds=time.perf_counter() - t0)
+    net.close()
+    return summary
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=128)
+    ap.add_argument("--heldout", type=int, default=16)
+    ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--order", type=int, default=1)
+    ap.add_argument("--max-iter", type=int, default=30000)
+    ap.add_argument("--criterion", type=float, default=-8.0)
+    ap.add_argument("--steps", type=int, default=30000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--deeper", type=float, default=3.0,
+                    help="decades beyond the criterion for the converged-coefficient comparison (0: skip)")
+    ap.add_argument("--json")
+    a = ap.parse_args(argv)
+    s = run(a.train, a.heldout, a.nx, a.nx // 3, a.nx // 3, a.order, a.max_iter, a.criterion, a.steps, a.batch, a.lr,
+            deeper=a.deeper)
+    print(f"[warmstart] held-out converged in both: {s['converged_both']}/{a.heldout}; cold/warm iterations "
+          f"{s['iterations_ratio']:.2f}x, time {s['time_ratio']:.2f}x; Cl/Cd rel diff at the criterion "
+          f"{s['max_cl_rel_diff']:.2e}/{s['max_cd_rel_diff']:.2e}, {a.deeper:g} decades deeper "
+          f"{s['max_cl_deep_rel_diff']}/{s['max_cd_deep_rel_diff']}", file=sys.stderr, flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(s, f, indent=1)
+    return s
+
+
+if __name__ == "__main__":
+    main()
