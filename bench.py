@@ -15,9 +15,16 @@ gradient + limiter + face states, or flux_update: residual + RK update + BCs
 on a stage with dissipation), algorithmic bytes per launch / average launch
 time from HIP events recorded on the context's stream during the timed region;
 "rooflines" carries the same figures for both kernels.
-cpu_baseline: the oracle (plain-C restatement, one core) on a bounded sample of
-the same workload: one of the 8 blocks (589,824 cells) for 4 RK iterations
-(about 10 s).
+cpu_baseline: the oracle (plain-C restatement of the reference's solver loop)
+on the host cores, on a bounded sample of the same workload: all 8 blocks, one
+thread per block (the reference's MPI decomposition), 4 RK iterations;
+cpu_baseline_1core: block 0 (589,824 cells) alone on one core, 4 iterations.
+
+Other workloads: --scheme ausm-lsq-bj --nx 384 --ny 128 --nz 128 --parts 1 1 1
+is the 37,748,736-cell single block with AUSM, least squares and
+Barth-Jespersen; --brick -S numbers the tets in a pseudo-random order (seed S)
+and --order rcm renumbers cells and faces in device memory by reverse
+Cuthill-McKee (the same results bit for bit).
 """
 from __future__ import annotations
 
@@ -31,28 +38,82 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Compulsory bytes per cell and RK stage of the whole second-order Roe/GG/VK
-# stage at this workload (K_A + K_B + K_C + per-iteration work / 5, below).
-MODEL_BYTES_PER_CELL_STAGE = 1110.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_kernels_bump192x64x64_traffic.json")
 
+# Schemes of the benchmark workloads: 4.7M-cell channel in 8 blocks (Roe,
+# Green-Gauss, Venkatakrishnan K=5, M=0.675) and the 37.7M-cell single block
+# (AUSM, least squares, Barth-Jespersen omega=1e-16, M=0.7, inlet angle 3 deg).
+SCHEMES = {
+    "roe-gg-vk": dict(mach=0.675, aoa=0.0, flux=0, gradient=0, limiter=1, k=5.0, omega=1e-16,
+                      label="2nd-order Roe + Green-Gauss + Venkatakrishnan K=5, M=0.675"),
+    "ausm-lsq-bj": dict(mach=0.7, aoa=3.0, flux=1, gradient=1, limiter=0, k=5.0, omega=1e-16,
+                        label="2nd-order AUSM + least squares + Barth-Jespersen omega=1e-16, M=0.7, 3 deg"),
+}
 
-def algorithmic_bytes(blocks):
-    """Algorithmic bytes per launch of the two stage kernels.
+
+def algorithmic_bytes(blocks, lsq=False):
+    """Algorithmic bytes per launch of the two stage kernels, and per cell-stage of the whole step.
 
     The compulsory-traffic model of this path: every array element counted once
     per kernel that must read or write it (W 40 B/cell, grad 120, lim 40, W0 40,
     conv+diss 80, dt 8, V 8; per face S 24 + rL, rR 48 + left/right 8 = 80 B).
-    interp is the gradient + limiter kernel K_A = 208(N+G) + 80F + 16N;
+    interp is the gradient + limiter kernel K_A = 208(N+G) + 80F + 16N with
+    Green-Gauss, 200(N+G) + 128N + 56F + 24G with least squares (weights 96 B,
+    cell-to-cell and cell-to-face lists 16 B each per cell; faces only for r);
     flux_update does the residual kernel K_B = 200(N+G) + 80F + 120N and the
-    update + BC kernel K_C = 176N + 112G of that model in one launch."""
+    update + BC kernel K_C = 176N + 112G of that model in one launch. Once per
+    iteration: saveW0, spectral radius, time step and monitors, 232N + 24F."""
     N = sum(b["nreal"] for b in blocks)
     G = sum(b["nall"] - b["nreal"] for b in blocks)
     F = sum(b["nface"] for b in blocks)
-    ka = 208 * (N + G) + 80 * F + 16 * N
+    ka = 200 * (N + G) + 128 * N + 56 * F + 24 * G if lsq else 208 * (N + G) + 80 * F + 16 * N
     kb = 200 * (N + G) + 80 * F + 120 * N
     kc = 176 * N + 112 * G
-    return {"interp": ka, "flux_update": kb + kc}, N, G, F
+    per_cell_stage = (ka + kb + kc + (232 * N + 24 * F) / 5) / N
+    return {"interp": ka, "flux_update": kb + kc}, per_cell_stage, N, G, F
+
+
+def cpu_baselines(a, sc, blocks):
+    """The oracle (plain-C restatement of the reference's solver loop) on the
+    host cores, on a bounded sample of the workload: all blocks of the mesh
+    with one thread per block in contiguous ranges (the reference's MPI
+    decomposition, one exchange per iteration), and block 0 alone on one core.
+    Meshes above 5M cells are sampled by the same scheme on the 96x32x32x6
+    channel in 8 blocks (589,824 cells)."""
+    import oracle as O
+    import euler3d_amd as E
+    sample = f"the same mesh ({sum(b['nreal'] for b in blocks)} cells in {len(blocks)} blocks)"
+    if sum(b["nreal"] for b in blocks) > 5_000_000:
+        m = E.Mesh.bump(96, 32, 32, 0.1, 4, (2, 2, 2))
+        blocks = [m.block(b) for b in range(m.nblocks)]
+        sample = "the same scheme on the 96x32x32x6 channel (589,824 cells in 8 blocks)"
+    prm = O.params(sc["mach"], sc["aoa"], flux=sc["flux"], order=2, gradient=sc["gradient"], limiter=sc["limiter"],
+                   k=sc["k"], omega=sc["omega"], stages=5, max_iter=10 ** 9, criterion=-1e9)
+    threads = max(1, min(len(blocks), os.cpu_count() or 1, 16))
+    S = O.Solver(blocks, prm)
+    S.init_freestream()
+    c0 = time.perf_counter()
+    S.iterate(a.cpu_iters, threads=threads)
+    c1 = time.perf_counter()
+    n_all = sum(b["nreal"] for b in blocks)
+    cpu = {"value": n_all * 5 * a.cpu_iters / (c1 - c0) / 1e6, "unit": "Mcell-RK-stage/s", "cores": threads,
+           "kind": "port",
+           "sample": f"oracle (plain-C restatement) on {sample}, one thread per block in contiguous ranges "
+                     f"({threads} threads), {a.cpu_iters} RK iteration(s) of 5 stages, {c1 - c0:.1f}s"}
+    print(f"[bench] cpu baseline {threads} threads: {cpu['value']:.3f} Mcell-RK-stage/s", file=sys.stderr,
+          flush=True)
+    b0 = blocks[0]
+    S = O.Solver([b0], prm, interfaces=False)
+    S.init_freestream()
+    c0 = time.perf_counter()
+    S.iterate(a.cpu_iters)
+    c1 = time.perf_counter()
+    cpu1 = {"value": b0["nreal"] * 5 * a.cpu_iters / (c1 - c0) / 1e6, "unit": "Mcell-RK-stage/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle on block 0 of {sample} ({b0['nreal']} cells), 1 thread, {a.cpu_iters} RK "
+                      f"iteration(s), {c1 - c0:.1f}s"}
+    print(f"[bench] cpu baseline 1 thread: {cpu1['value']:.3f} Mcell-RK-stage/s", file=sys.stderr, flush=True)
+    return cpu, cpu1
 
 
 def main():
@@ -64,6 +125,12 @@ def main():
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--parts", type=int, nargs=3, default=[2, 2, 2])
+    ap.add_argument("--scheme", choices=sorted(SCHEMES), default="roe-gg-vk")
+    ap.add_argument("--brick", type=int, default=4,
+                    help="cell numbering of the generated mesh: bricks of brick^3 hexes (0 lexicographic), "
+                         "or < 0 for tets in a pseudo-random order seeded by -brick")
+    ap.add_argument("--order", choices=["reference", "rcm"], default="reference",
+                    help="device memory order of cells and faces (results are the same bit for bit)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -77,15 +144,20 @@ def main():
 
     import euler3d_amd as E
 
+    sc = SCHEMES[a.scheme]
+    print(f"[bench] building the {a.nx}x{a.ny}x{a.nz}x6 mesh in {a.parts} blocks", file=sys.stderr, flush=True)
     t_mesh = time.perf_counter()
-    mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, 4, tuple(a.parts))
+    mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, a.brick, tuple(a.parts))
     blocks = [mesh.block(b) for b in range(mesh.nblocks)]
     t_mesh = time.perf_counter() - t_mesh
-    alg, N, G, F = algorithmic_bytes(blocks)
-    params = E.make_params(0.675, 0.0, flux=E.FLUX_ROE, order=2, gradient=E.GRAD_GREEN_GAUSS,
-                           limiter=E.LIM_VENKATAKRISHNAN, k=5.0, stages=5, max_iter=10 ** 9, criterion=-1e9)
+    alg, model_bytes, N, G, F = algorithmic_bytes(blocks, lsq=sc["gradient"] == 1)
+    params = E.make_params(sc["mach"], sc["aoa"], flux=sc["flux"], order=2, gradient=sc["gradient"],
+                           limiter=sc["limiter"], k=sc["k"], omega=sc["omega"], stages=5, max_iter=10 ** 9,
+                           criterion=-1e9)
     ctx = E.Context(0)
     ctx.set_params(params)
+    ctx.set_ordering(E.ORDER_RCM if a.order == "rcm" else E.ORDER_REFERENCE)
+    print(f"[bench] mesh built in {t_mesh:.1f}s; uploading", file=sys.stderr, flush=True)
     ctx.upload(mesh)
     ctx.init_freestream()
     print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s); "
@@ -106,7 +178,8 @@ def main():
         ms, n = ctx.kernel_time(name)
         kt[name] = {"avg_ms": ms, "launches": n}
     measured = {}
-    if os.path.exists(TRAFFIC_FILE) and (a.nx, a.ny, a.nz, tuple(a.parts)) == (192, 64, 64, (2, 2, 2)):
+    if os.path.exists(TRAFFIC_FILE) and (a.nx, a.ny, a.nz, tuple(a.parts), a.scheme, a.brick, a.order) == \
+            (192, 64, 64, (2, 2, 2), "roe-gg-vk", 4, "reference"):
         with open(TRAFFIC_FILE) as f:
             measured = json.load(f)["hbm_bytes_per_launch"]
     rooflines = {}
@@ -134,22 +207,9 @@ def main():
         hip.hipFree(dst)
         print(f"[bench] calibration copy of 1 GiB: {'done' if ok else 'FAILED'}", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu, cpu1 = None, None
     if not a.no_cpu_baseline:
-        import oracle as O
-        b0 = blocks[0]
-        S = O.Solver([b0], O.params(0.675, 0.0, flux=0, order=2, gradient=0, limiter=1, k=5.0, stages=5,
-                                    max_iter=10 ** 9, criterion=-1e9), interfaces=False)
-        S.init_freestream()
-        c0 = time.perf_counter()
-        S.iterate(a.cpu_iters)
-        c1 = time.perf_counter()
-        cpu = {"value": b0["nreal"] * stages * a.cpu_iters / (c1 - c0) / 1e6, "unit": "Mcell-RK-stage/s",
-               "cores": 1, "kind": "port",
-               "sample": f"oracle (plain-C restatement, 1 thread) on block 0 of the same mesh "
-                         f"({b0['nreal']} cells), {a.cpu_iters} RK iteration(s) of 5 stages, "
-                         f"{c1 - c0:.1f}s"}
-        print(f"[bench] cpu baseline {cpu['value']:.3f} Mcell-RK-stage/s", file=sys.stderr, flush=True)
+        cpu, cpu1 = cpu_baselines(a, sc, blocks)
 
     out = {
         "metric": "Mcell-RK-stage/s (FP64, %HBM roof) + time to 8-order residual drop, warm vs cold",
@@ -157,14 +217,17 @@ def main():
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"bump channel {a.nx}x{a.ny}x{a.nz}x6 tets ({N} cells), "
-                               f"{mesh.nblocks} blocks on one device, 2nd-order Roe + Green-Gauss + "
-                               "Venkatakrishnan K=5, M=0.675, 5-stage RK, CFL 1.5",
-                   "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages},
-        "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / MODEL_BYTES_PER_CELL_STAGE / 1e6),
+                               f"{mesh.nblocks} block(s) on one device, {sc['label']}, 5-stage RK, CFL 1.5",
+                   "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages,
+                   "cell_numbering": "scrambled" if a.brick < 0 else f"brick {a.brick}",
+                   "memory_order": a.order},
+        "model_bytes_per_cell_stage": model_bytes,
+        "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / model_bytes / 1e6),
         "roofline": rooflines[dominant],
         "rooflines": rooflines,
         "kernels": kt,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
     }
     print(json.dumps(out))
 

@@ -19,6 +19,7 @@ FLUX_ROE, FLUX_AUSM = 0, 1
 GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
 LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1
 ARR_W, ARR_W0, ARR_CONV, ARR_DISS, ARR_GRAD, ARR_LIM, ARR_DT, ARR_SPECRAD = range(8)
+ORDER_REFERENCE, ORDER_RCM = 0, 1
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
@@ -128,6 +129,7 @@ def lib():
             "e3d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(BatchStatus)]),
             "e3d_batch_history": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor), C.c_int]),
             "e3d_set_tile_order": (C.c_int, [C.c_void_p, C.c_int]),
+            "e3d_set_ordering": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
                                          C.POINTER(C.c_void_p)]),
             "e3d_mlp_destroy": (None, [C.c_void_p]),
@@ -313,6 +315,10 @@ class Context:
 
     def set_tile_order(self, xcd_contiguous):
         self._c(lib().e3d_set_tile_order(self._h, 1 if xcd_contiguous else 0))
+
+    def set_ordering(self, mode):
+        """Memory order of the next upload: ORDER_REFERENCE or ORDER_RCM."""
+        self._c(lib().e3d_set_ordering(self._h, mode))
 
     def init_freestream(self):
         self._c(lib().e3d_init_freestream(self._h))

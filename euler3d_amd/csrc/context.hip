@@ -13,6 +13,7 @@
 #include "abi_internal.hpp"
 #include "device.hpp"
 #include "mesh_internal.hpp"
+#include "ordering.hpp"
 #include "physics.hpp"
 
 namespace e3d {
@@ -99,6 +100,9 @@ struct e3d_ctx {
     double* graph_W = nullptr;   // W buffer the graph was captured with
     bool capturing = false;
     int tile_map = 1;            // tile order of the stage kernels (e3d_set_tile_order)
+    int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
+    // per block: device position of each real cell and face (empty: the reference's numbering)
+    std::vector<std::vector<int>> cell_pos, face_pos;
     // monitors
     int iter = 0;
     bool started = false;      // initial exchange done
@@ -171,9 +175,12 @@ int upload(e3d_ctx* c, T** p, const std::vector<T>& h) {
     return E3D_OK;
 }
 
+// Device position of a block's cell (reference-numbered, real or ghost) and face.
 int global_cell(const e3d_ctx* c, int b, int local) {
-    return local < c->nreal[b] ? c->roff[b] + local : c->m.NRp + c->goff[b] + (local - c->nreal[b]);
+    if (local >= c->nreal[b]) return c->m.NRp + c->goff[b] + (local - c->nreal[b]);
+    return c->roff[b] + (c->cell_pos[b].empty() ? local : c->cell_pos[b][local]);
 }
+int global_face(const e3d_ctx* c, int b, int f) { return c->foff[b] + (c->face_pos[b].empty() ? f : c->face_pos[b][f]); }
 
 Range block_range(const e3d_ctx* c, int b) {
     if (b < 0) return Range{0, c->m.NRp, 0, c->m.NG};
@@ -436,6 +443,10 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         for (int q = 0; q < B.nreal; ++q) nslot = std::max(nslot, B.cell_off[q + 1] - B.cell_off[q]);
     }
     if (nslot > kMaxSlots) return set_error(c, E3D_EINVAL, "cells with more than 6 faces are not supported");
+    c->cell_pos.assign(nb, {});
+    c->face_pos.assign(nb, {});
+    if (c->ordering != ORDER_REFERENCE)
+        for (int b = 0; b < nb; ++b) block_layout(*src[b].B, c->ordering, c->cell_pos[b], c->face_pos[b]);
     if (NRp + NG >= (1L << 31) - 1 || NF >= (1L << 31) - 1)
         return set_error(c, E3D_EINVAL, "mesh too large for 32-bit cell indices");
     DMesh& m = c->m;
@@ -465,7 +476,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         ntile[b] = (B.nreal + kWG - 1) / kWG;
         for (int t = 0; t < ntile[b]; ++t) wg_block[tile0[b] + t] = b;
         for (int f = 0; f < B.nface; ++f) {
-            const size_t g = c->foff[b] + (size_t)f;
+            const size_t g = global_face(c, b, f);
             for (int d = 0; d < 3; ++d) {
                 S[d * (size_t)NF + g] = B.S[3 * f + d];
                 rL[d * (size_t)NF + g] = B.rL[3 * f + d];
@@ -475,7 +486,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
             area[g] = B.area[f];
         }
         for (int q = 0; q < B.nreal; ++q) {
-            const int gc = c->roff[b] + q;
+            const int gc = global_cell(c, b, q);
             nf[gc] = B.cell_off[q + 1] - B.cell_off[q];
             vol[gc] = B.volume[q];
             for (int k = 0; k < nf[gc]; ++k) {
@@ -483,7 +494,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
                 const bool right = B.face2cell[2 * f] != q;
                 const int other = B.cell2cell[B.cell_off[q] + k];
                 nbr[(size_t)k * NR + gc] = global_cell(c, b, other);
-                fid[(size_t)k * NR + gc] = (c->foff[b] + f) | (right ? (int)0x80000000u : 0);
+                fid[(size_t)k * NR + gc] = global_face(c, b, f) | (right ? (int)0x80000000u : 0);
                 for (int d = 0; d < 3; ++d)
                     lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
                 // geometry copies: S, own-side r, S/|S| and |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz)
@@ -506,7 +517,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
                     for (int j = B.cell_off[other]; j < B.cell_off[other + 1]; ++j)
                         if (B.cell2face[j] == f) ko = j - B.cell_off[other];
                     if (ko < 0) return set_error(c, E3D_EINVAL, "upload: inconsistent face connectivity");
-                    rec = (size_t)ko * NR + (size_t)(c->roff[b] + other);
+                    rec = (size_t)ko * NR + (size_t)global_cell(c, b, other);
                 } else {
                     rec = ghost_record(nslot, (int)NR, c->goff[b] + (other - B.nreal));
                 }
@@ -520,7 +531,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
             for (int gl : B.bnds[j].ghosts) {
                 const int gi = c->goff[b] + (gl - B.nreal);
                 g_in[gi] = global_cell(c, b, B.cell2cell[B.cell_off[gl]]);
-                g_face[gi] = c->foff[b] + B.cell2face[B.cell_off[gl]];
+                g_face[gi] = global_face(c, b, B.cell2face[B.cell_off[gl]]);
                 for (int d = 0; d < 3; ++d) g_r[(size_t)d * NG + gi] = B.rR[3 * (size_t)B.cell2face[B.cell_off[gl]] + d];
                 int kind = 0;
                 if (real_bnd) {
@@ -546,7 +557,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
             const int o = src[b].base + F.itf_origin_block[i];
             for (int q = F.itf_off[i]; q < F.itf_off[i + 1]; ++q, ++k) {
                 itf_dst.push_back(global_cell(c, b, c->ghost_count[b] + k));
-                itf_org.push_back(c->roff[o] + F.itf_origin_cells[q]);
+                itf_org.push_back(global_cell(c, o, F.itf_origin_cells[q]));
             }
         }
     }
@@ -672,11 +683,21 @@ int copy_array(e3d_ctx* c, int b, int which, double* host, bool to_device) {
     const int nr = c->nreal[b], ng = c->nall[b] - c->nreal[b];
     const size_t n_host = real_only ? nr : (size_t)c->nall[b];
     const hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    // real cells in device order: staged through a host buffer and permuted to
+    // or from the reference's numbering
+    const std::vector<int>& pos = c->cell_pos[b];
+    std::vector<double> stage(pos.empty() ? 0 : (size_t)nc * nr);
     for (int q = 0; q < nc; ++q) {
         double* d_real = base + q * stride + c->roff[b];
         double* h = host + q * n_host;
-        if (to_device) HIP_TRY(c, hipMemcpyAsync(d_real, h, sizeof(double) * nr, kind, c->stream));
-        else HIP_TRY(c, hipMemcpyAsync(h, d_real, sizeof(double) * nr, kind, c->stream));
+        double* hr = pos.empty() ? h : stage.data() + (size_t)q * nr;
+        if (to_device) {
+            if (!pos.empty())
+                for (int i = 0; i < nr; ++i) hr[pos[i]] = h[i];
+            HIP_TRY(c, hipMemcpyAsync(d_real, hr, sizeof(double) * nr, kind, c->stream));
+        } else {
+            HIP_TRY(c, hipMemcpyAsync(hr, d_real, sizeof(double) * nr, kind, c->stream));
+        }
         if (!real_only && ng > 0) {
             double* d_g = base + q * stride + c->m.NRp + c->goff[b];
             if (to_device) HIP_TRY(c, hipMemcpyAsync(d_g, h + nr, sizeof(double) * ng, kind, c->stream));
@@ -694,6 +715,12 @@ int copy_array(e3d_ctx* c, int b, int which, double* host, bool to_device) {
         }
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!to_device && !pos.empty())
+        for (int q = 0; q < nc; ++q) {
+            const double* hr = stage.data() + (size_t)q * nr;
+            double* h = host + q * n_host;
+            for (int i = 0; i < nr; ++i) h[i] = hr[pos[i]];
+        }
     return E3D_OK;
 }
 }  // namespace
@@ -1106,6 +1133,15 @@ int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
         HIP_TRY(c, hipMemcpy(rows, (MonRow*)c->d_hist + (size_t)i * c->hist_cap, sizeof(MonRow) * n,
                              hipMemcpyDeviceToHost));
     return n;
+}
+
+static_assert(E3D_ORDER_REFERENCE == ORDER_REFERENCE && E3D_ORDER_RCM == ORDER_RCM, "ordering modes");
+int e3d_set_ordering(e3d_ctx* c, int mode) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    if (mode != E3D_ORDER_REFERENCE && mode != E3D_ORDER_RCM)
+        return set_error(c, E3D_EINVAL, "e3d_set_ordering: unknown mode " + std::to_string(mode));
+    c->ordering = mode;
+    return E3D_OK;
 }
 
 int e3d_set_tile_order(e3d_ctx* c, int xcd_contiguous) {

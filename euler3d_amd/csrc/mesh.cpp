@@ -125,6 +125,30 @@ MultiBlockMesh make_bump_channel(const BumpParams& p) {
         }
     }
 
+    if (p.brick < 0) {
+        // A badly numbered mesh (memory-order studies): the tets in a
+        // pseudo-random order, Fisher-Yates driven by splitmix64 seeded with -brick.
+        uint64_t st = (uint64_t)(-(long long)p.brick);
+        auto next = [&st]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        std::vector<size_t> perm(ntet);
+        for (size_t g = 0; g < ntet; ++g) perm[g] = g;
+        for (size_t g = ntet; g > 1; --g) std::swap(perm[g - 1], perm[next() % g]);
+        std::vector<int> nodes2(4 * ntet), block2(ntet);
+        for (size_t g = 0; g < ntet; ++g) {
+            for (int q = 0; q < 4; ++q) nodes2[4 * g + q] = tet_nodes[4 * perm[g] + q];
+            block2[g] = tet_block[perm[g]];
+        }
+        tet_nodes.swap(nodes2);
+        tet_block.swap(block2);
+        std::fill(block_ntet.begin(), block_ntet.end(), 0);
+        for (size_t g = 0; g < ntet; ++g) tet_local[g] = block_ntet[tet_block[g]]++;
+    }
+
     // Physical boundary triangles, with the marker they belong to.
     struct BTri { int n[3]; BcType type; };
     std::vector<BTri> btris;

@@ -71,7 +71,8 @@ struct HostBlock {
 struct BumpParams {
     int nx = 48, ny = 16, nz = 16;
     double h = 0.1;                     // bump thickness / chord
-    int brick = 4;                      // hex brick edge for cell ordering (0 = lexicographic)
+    int brick = 4;                      // hex brick edge for cell ordering (0 = lexicographic,
+                                        // < 0: tets in a pseudo-random order seeded by -brick)
     int px = 1, py = 1, pz = 1;         // partition into px*py*pz blocks
 };
 

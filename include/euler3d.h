@@ -52,7 +52,9 @@ typedef struct e3d_mesh e3d_mesh;
 /* Bump channel [0,3]x[0,1]x[0,1], circular-arc bump of thickness h on the
  * lower wall for 1 <= x <= 2, nx*ny*nz hexes split into 6 tets each.
  * WALL = lower wall, SYMMETRY = top and sides, FARFIELD = inlet and outlet.
- * Cells are numbered brick by brick (brick^3 hexes, 0 = lexicographic);
+ * Cells are numbered brick by brick (brick^3 hexes, 0 = lexicographic;
+ * brick < 0: tets in a pseudo-random order seeded by -brick, a badly numbered
+ * mesh for memory-order studies);
  * px*py*pz > 1 splits the hexes into that many blocks with CONNEXION markers. */
 typedef struct e3d_bump_desc {
     int nx, ny, nz;
@@ -255,6 +257,22 @@ int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
 /* Workgroup-to-tile order of the stage kernels: 1 (default) gives each of the
  * eight XCDs a contiguous range of cell tiles, 0 maps workgroup i to tile i. */
 int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
+
+/* Memory order of the cells and faces of the next uploaded mesh or batch.
+ * E3D_ORDER_REFERENCE (default) keeps the reference's numbering
+ * (BlockBuilder/src/BlockBuilder.cpp:147-458, cells as the mesh file lists them,
+ * faces by first appearance). E3D_ORDER_RCM places each block's real cells in
+ * reverse Cuthill-McKee order and its faces by first appearance over them, so
+ * that neighbour gathers stay local whatever order the mesh file came in. Only
+ * addresses change: every cell keeps its faces in the reference's order, so
+ * every per-cell value is bit for bit the same; the residual norms, summed
+ * over cells in memory order, agree to rounding. Arrays are still read and
+ * written in the reference's numbering (e3d_get_array, e3d_set_array). The
+ * reference has no counterpart (its Block arrays are in file order,
+ * DataStructure/head/Block.h:17-60). */
+#define E3D_ORDER_REFERENCE 0
+#define E3D_ORDER_RCM 1
+int e3d_set_ordering(e3d_ctx* ctx, int mode);
 
 /* ------------------------------------------------------ learned warm start
  * A coordinate MLP that predicts an initial flow field, replacing the uniform

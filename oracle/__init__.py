@@ -85,6 +85,9 @@ def lib():
         L.e3do_iterate.argtypes = [bp, C.c_int, C.POINTER(Interface), C.c_int, C.POINTER(Params),
                                    C.c_int, _ip, _dp, C.POINTER(Monitor)]
         L.e3do_iterate.restype = C.c_int
+        L.e3do_iterate_mt.argtypes = [bp, C.c_int, C.POINTER(Interface), C.c_int, C.POINTER(Params),
+                                      C.c_int, _ip, _dp, C.POINTER(Monitor), C.c_int]
+        L.e3do_iterate_mt.restype = C.c_int
         _lib = L
     return _lib
 
@@ -174,10 +177,12 @@ class Solver:
         for i, b in enumerate(self.blocks):
             lib().e3do_init_flow(C.byref(self.arr[i]), _p(self.fs, _dp))
 
-    def iterate(self, n):
+    def iterate(self, n, threads=1):
+        """n iterations of Solver::solve; threads > 1 spreads each iteration's
+        blocks over that many threads in contiguous ranges (same results)."""
         hist = (Monitor * max(n, 1))()
-        done = lib().e3do_iterate(self.arr, len(self.blocks), self.itf, self.n_itf, C.byref(self.p), n,
-                                  C.byref(self.it), _p(self.rms0, _dp), hist)
+        done = lib().e3do_iterate_mt(self.arr, len(self.blocks), self.itf, self.n_itf, C.byref(self.p), n,
+                                     C.byref(self.it), _p(self.rms0, _dp), hist, threads)
         return [hist[i].as_dict() for i in range(done)]
 
 

@@ -696,14 +696,21 @@ void e3do_exchange(e3do_block* blocks, const e3do_interface* itf, int n_itf) {
 /* Solver::solve (Solver/src/Solver.cpp:36-98) with PostProcessing::computeFlowData / process
  * (PostProcessing/src/PostProcessing.cpp:198-308) and the serial getGlobal sum
  * (MPI/src/BlockCommunicator.cpp:341-369). */
-int e3do_iterate(e3do_block* blocks, int nb, const e3do_interface* itf, int n_itf, const e3do_params* p,
-                 int n, int* it, double rms0[5], e3do_monitor* history) {
+/* Solver::solve (Solver/src/Solver.cpp:36-98). With nthreads > 1 the blocks of
+ * an iteration are split into contiguous ranges, one per thread, as the
+ * reference's MPI build gives each rank a contiguous block range
+ * (MPI/src/BlockCommunicator.cpp:284-294); the exchange and the sums over
+ * blocks stay sequential, so the result does not depend on nthreads. */
+int e3do_iterate_mt(e3do_block* blocks, int nb, const e3do_interface* itf, int n_itf, const e3do_params* p,
+                    int n, int* it, double rms0[5], e3do_monitor* history, int nthreads) {
     double fs[5];
     e3do_freestream(p, fs);
     if (*it == 0) e3do_exchange(blocks, itf, n_itf);
     int done = 0;
     double* brms = (double*)__builtin_alloca(sizeof(double) * 10 * (size_t)nb);
+    if (nthreads < 1) nthreads = 1;
     for (int k = 0; k < n; ++k) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
         for (int i = 0; i < nb; ++i) {
             e3do_block* b = &blocks[i];
             e3do_save_w0(b, p->gamma);
@@ -737,4 +744,9 @@ int e3do_iterate(e3do_block* blocks, int nb, const e3do_interface* itf, int n_it
         if (m.stop) break;
     }
     return done;
+}
+
+int e3do_iterate(e3do_block* blocks, int nb, const e3do_interface* itf, int n_itf, const e3do_params* p,
+                 int n, int* it, double rms0[5], e3do_monitor* history) {
+    return e3do_iterate_mt(blocks, nb, itf, n_itf, p, n, it, rms0, history, 1);
 }

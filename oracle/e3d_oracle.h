@@ -75,6 +75,10 @@ void e3do_exchange(e3do_block* blocks, const e3do_interface* itf, int n_itf);
  * Returns the number of iterations run. */
 int e3do_iterate(e3do_block* blocks, int nb, const e3do_interface* itf, int n_itf, const e3do_params* p,
                  int n, int* it, double rms0[5], e3do_monitor* history);
+/* The same with the blocks of each iteration spread over nthreads threads
+ * (contiguous block ranges); identical results for any nthreads. */
+int e3do_iterate_mt(e3do_block* blocks, int nb, const e3do_interface* itf, int n_itf, const e3do_params* p,
+                    int n, int* it, double rms0[5], e3do_monitor* history, int nthreads);
 
 #ifdef __cplusplus
 }

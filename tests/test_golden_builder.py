@@ -71,3 +71,23 @@ def test_multiblock_partition_topology():
             assert o != d
             cells = b["itf_origin_cells"][b["itf_off"][i]:b["itf_off"][i + 1]]
             assert cells.min() >= 0 and cells.max() < blocks[o]["nreal"]
+
+
+def test_scrambled_bump_is_a_renumbering():
+    """brick < 0 lists the same tets in a pseudo-random order: same counts,
+    boundary lists and total volume; the same multiset of cell centroids; a
+    different cell order; and the same order for the same seed."""
+    import euler3d_amd as E
+    a = E.Mesh.bump(6, 3, 3, 0.1, 0, (1, 1, 1)).block(0)
+    b = E.Mesh.bump(6, 3, 3, 0.1, -11, (1, 1, 1)).block(0)
+    b2 = E.Mesh.bump(6, 3, 3, 0.1, -11, (1, 1, 1)).block(0)
+    N = a["nreal"]
+    assert (b["nreal"], b["nall"], b["nface"]) == (N, a["nall"], a["nface"])
+    assert b["n_real_boundaries"] == a["n_real_boundaries"] and len(b["wall_faces"]) == len(a["wall_faces"])
+    ca = a["ccenter"].reshape(-1, 3)[:N]
+    cb = b["ccenter"].reshape(-1, 3)[:N]
+    key = lambda c: np.lexsort(np.round(c, 12).T[::-1])
+    assert np.allclose(ca[key(ca)], cb[key(cb)], rtol=0, atol=1e-12)
+    assert not np.allclose(ca, cb)
+    assert abs(a["volume"][:N].sum() - b["volume"][:N].sum()) < 1e-12
+    assert (b["ccenter"] == b2["ccenter"]).all()

@@ -94,3 +94,20 @@ def test_oracle_c1_config_200_iterations():
     rows = S.run(meta["iters"], on_iter=on_iter)
     assert len(rows) == meta["iters"]
     g.check("monitor", np.concatenate([monitor_row(m) for m in rows]), "monitor history")
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle spread over threads (contiguous block ranges, as the
+    reference's MPI ranks) gives the same bits as the sequential loop."""
+    import euler3d_amd as E
+    mesh = E.Mesh.bump(12, 4, 4, 0.1, 4, (2, 2, 1))
+    geoms = [mesh.block(b) for b in range(mesh.nblocks)]
+    runs = []
+    for threads in (1, 3):
+        S = O.Solver(geoms, O.params(0.675, 1.0, flux=0, order=2, gradient=0, limiter=1, k=5.0, stages=5,
+                                     max_iter=100, criterion=-15.0))
+        S.init_freestream()
+        hist = S.iterate(3, threads=threads)
+        runs.append((np.concatenate([b.W for b in S.blocks]), [h["rms"] for h in hist]))
+    assert (runs[0][0].view(np.int64) == runs[1][0].view(np.int64)).all()
+    assert runs[0][1] == runs[1][1]

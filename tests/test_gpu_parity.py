@@ -23,9 +23,10 @@ ARR = {"W": E.ARR_W, "grad": E.ARR_GRAD, "lim": E.ARR_LIM, "conv": E.ARR_CONV, "
 MON_RTOL = 1e-12
 
 
-def gpu_context(mesh, params):
+def gpu_context(mesh, params, ordering=E.ORDER_REFERENCE):
     ctx = E.Context(0)
     ctx.set_params(params)
+    ctx.set_ordering(ordering)
     ctx.upload(mesh)
     ctx.init_freestream()
     return ctx
@@ -36,13 +37,19 @@ def close_rel(a, b, rtol=MON_RTOL):
     return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-300)) or np.allclose(a, b, rtol=rtol, atol=1e-280)
 
 
+ORDERINGS = {"reference_order": E.ORDER_REFERENCE, "rcm_order": E.ORDER_RCM}
+
+
+@pytest.mark.parametrize("ordering", list(ORDERINGS))
 @pytest.mark.parametrize("case", SMALL_FIXTURES)
-def test_granular_operators_match_reference_bitwise(case):
+def test_granular_operators_match_reference_bitwise(case, ordering):
+    """In either memory order (cells and faces renumbered on the device by
+    reverse Cuthill-McKee, or not) every array equals the reference's."""
     g = Golden(case)
     meta = g.meta
     mesh = mesh_for(g)
     nb = mesh.nblocks
-    ctx = gpu_context(mesh, product_params(meta))
+    ctx = gpu_context(mesh, product_params(meta), ORDERINGS[ordering])
     k = meta["stage_iter"]
     ctx.op("exchange")                      # Solver.cpp:48
     ref_mon = g.monitor()
@@ -119,8 +126,8 @@ def test_c1_config_fused_200_iterations():
     ctx.close()
 
 
-def run_both(mesh, meta, n):
-    ctx = gpu_context(mesh, product_params(meta))
+def run_both(mesh, meta, n, ordering=E.ORDER_REFERENCE):
+    ctx = gpu_context(mesh, product_params(meta), ordering)
     hist = ctx.iterate(n)
     geoms = [mesh.block(b) for b in range(mesh.nblocks)]
     S = O.Solver(geoms, oracle_params(meta))
@@ -148,11 +155,12 @@ def assert_monitors_close(hist, rh):
                          [r["cl"], r["cd"], r["cmx"], r["cmy"], r["cmz"]]), a["iter"]
 
 
+@pytest.mark.parametrize("ordering", list(ORDERINGS))
 @pytest.mark.parametrize("case", SMALL_FIXTURES)
-def test_fused_iterations_match_oracle(case):
+def test_fused_iterations_match_oracle(case, ordering):
     g = Golden(case)
     mesh = mesh_for(g)
-    ctx, hist, S, rh = run_both(mesh, g.meta, g.meta["iters"])
+    ctx, hist, S, rh = run_both(mesh, g.meta, g.meta["iters"], ORDERINGS[ordering])
     assert_states_equal(ctx, S)
     assert_monitors_close(hist, rh)
     # and against the reference's own monitor history
