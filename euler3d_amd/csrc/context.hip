@@ -29,8 +29,8 @@ void launch_bc(const Range&, const DMesh&, const DState&, double, const double*,
 void launch_exchange(int, const int*, const int*, int, double*, double*, double*, int, hipStream_t);
 void launch_rms_partials(const DMesh&, const DState&, hipStream_t);
 void launch_smooth(const Range&, const DMesh&, const DState&, hipStream_t);
-void launch_block_monitors(int, const int*, const int*, const int*, const int*, const int*, const int*,
-                           const double*, const DMesh&, const double*, const double*, const double*, double*,
+void launch_block_monitors(int, int, const int*, const int*, const int*, const int*, const int*, const int*,
+                           const double*, const DMesh&, const double*, const double*, const double*, double*, double*,
                            hipStream_t);
 void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
 void launch_batch_reduce(int, const int*, const double*, const void*, int*, double*, int*, int*, void*, int, int*,
@@ -76,6 +76,8 @@ struct e3d_ctx {
         *d_wall_face = nullptr;
     double* d_fcenter = nullptr;
     double* d_mon = nullptr;   // [nb][16]
+    double* d_mpart = nullptr; // [nb][mon_split][10] first-level monitor sums
+    int mon_split = 1;         // workgroups per block of the first monitor level
     double* h_mon = nullptr;   // pinned
     // instances: one mesh is instance 0 with the conditions of params; a batch
     // (e3d_upload_batch) has one instance per mesh with its own conditions
@@ -235,8 +237,9 @@ void ev_collect(e3d_ctx* c) {
 int finish_monitor(e3d_ctx* c, e3d_monitor* out) {
     size_t ev = 0;
     ev_begin(c, KI_MONITORS, &ev);
-    launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
-                          c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mon, c->stream);
+    launch_block_monitors(c->nb, c->mon_split, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost,
+                          c->d_wall_face, c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mpart, c->d_mon,
+                          c->stream);
     ev_end(c, ev);
     int rc = launch_status(c);
     if (rc) return rc;
@@ -447,8 +450,8 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     c->face_pos.assign(nb, {});
     if (c->ordering != ORDER_REFERENCE)
         for (int b = 0; b < nb; ++b) block_layout(*src[b].B, c->ordering, c->cell_pos[b], c->face_pos[b]);
-    if (NRp + NG >= (1L << 31) - 1 || NF >= (1L << 31) - 1)
-        return set_error(c, E3D_EINVAL, "mesh too large for 32-bit cell indices");
+    if (NRp + NG >= (1L << 29) || NF >= (1L << 31) - 1)  // kernels address cell arrays with 32-bit byte offsets
+        return set_error(c, E3D_EINVAL, "mesh too large: at most 2^29 cells including ghosts");
     DMesh& m = c->m;
     m.NRp = (int)NRp;
     m.NG = (int)NG;
@@ -465,7 +468,10 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     std::vector<int> itf_dst, itf_org;
     // per-slot face geometry of the fused path
     const size_t NS = (size_t)nslot * NR;
-    std::vector<double> sS(3 * NS, 0.0), sR(3 * NS, 0.0), sN(3 * NS, 0.0), sA(NS, 1.0), g_r(3 * (size_t)NG, 0.0);
+    std::vector<double> sSR(6 * NS, 0.0), sNA(4 * NS, 0.0), g_r(3 * (size_t)NG, 0.0);
+    for (size_t q = 0; q < NS; ++q) sNA[2 * (NS + q) + 1] = 1.0;  // |S| of padding slots
+    // component j (0..5 of sSR, 0..3 of sNA) of slot k of cell gc: pair j/2, half j%2
+    auto pair_at = [&](int j, int k, int gcell) { return 2 * (((size_t)(j / 2) * nslot + k) * NR + gcell) + j % 2; };
     std::vector<int> mate(NS, 0);
     std::vector<int> g_blk(NG, 0);
     for (int b = 0; b < nb; ++b) {
@@ -505,11 +511,11 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
                 const double sv[3] = {sx, sy, sz};
                 for (int d = 0; d < 3; ++d) {
                     // oriented away from this cell, as the gradient uses it (GreenGauss.cpp:104-118)
-                    sS[((size_t)d * nslot + k) * NR + gc] = right ? sv[d] * -1.0 : sv[d];
-                    sR[((size_t)d * nslot + k) * NR + gc] = rv[d];
-                    sN[((size_t)d * nslot + k) * NR + gc] = sv[d] / norm;
+                    sSR[pair_at(d, k, gc)] = right ? sv[d] * -1.0 : sv[d];
+                    sSR[pair_at(3 + d, k, gc)] = rv[d];
+                    sNA[pair_at(d, k, gc)] = sv[d] / norm;
                 }
-                sA[(size_t)k * NR + gc] = norm;
+                sNA[pair_at(3, k, gc)] = norm;
                 // face-state record of the other cell on this face
                 size_t rec;
                 if (other < B.nreal) {
@@ -562,11 +568,19 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         }
     }
     c->n_itf = (int)itf_dst.size();
+    // monitor reduction: enough workgroups per block that the largest block's
+    // tiles and wall faces are spread over the chip (at most 64 per block)
+    {
+        int most = 1;
+        for (int b = 0; b < nb; ++b)
+            most = std::max(most, std::max(ntile[b], wall_off[b + 1] - wall_off[b]));
+        c->mon_split = std::min(64, std::max(1, (most + 4 * kWG - 1) / (4 * kWG)));
+    }
     int rc = 0;
     const int* cnf;
     (void)cnf;
     int *d_nf, *d_nbr, *d_fid, *d_gin, *d_gface, *d_gkind, *d_wgb, *d_mate;
-    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw, *d_sS, *d_sR, *d_sN, *d_sA, *d_gr;
+    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw, *d_sSR, *d_sNA, *d_gr;
     if ((rc = upload(c, &d_nf, nf)) || (rc = upload(c, &d_nbr, nbr)) || (rc = upload(c, &d_fid, fid)) ||
         (rc = upload(c, &d_gin, g_in)) || (rc = upload(c, &d_gface, g_face)) || (rc = upload(c, &d_gkind, g_kind)) ||
         (rc = upload(c, &d_wgb, wg_block)) || (rc = upload(c, &d_S, S)) || (rc = upload(c, &d_area, area)) ||
@@ -576,8 +590,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         (rc = upload(c, &c->d_ntile, ntile)) || (rc = upload(c, &c->d_nreal, c->nreal)) ||
         (rc = upload(c, &c->d_wall_off, wall_off)) || (rc = upload(c, &c->d_wall_ghost, wall_ghost)) ||
         (rc = upload(c, &c->d_wall_face, wall_face)) || (rc = upload(c, &d_mate, mate)) ||
-        (rc = upload(c, &d_sS, sS)) || (rc = upload(c, &d_sR, sR)) || (rc = upload(c, &d_sN, sN)) ||
-        (rc = upload(c, &d_sA, sA)) || (rc = upload(c, &d_gr, g_r)) || (rc = upload(c, &c->d_blk_inst, c->blk_inst)) ||
+        (rc = upload(c, &d_sSR, sSR)) || (rc = upload(c, &d_sNA, sNA)) || (rc = upload(c, &d_gr, g_r)) || (rc = upload(c, &c->d_blk_inst, c->blk_inst)) ||
         (rc = upload(c, &c->d_g_blk, g_blk)) || (rc = upload(c, &c->d_inst_b0, c->inst_b0)) ||
         (rc = dalloc(c, &c->d_inst_fs, 5 * (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_blk_qcmac, (size_t)nb)) ||
         (rc = dalloc(c, &c->d_active, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_frozen, (size_t)c->n_inst)) ||
@@ -589,7 +602,9 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     }
     m.nf = d_nf; m.nbr = d_nbr; m.fid = d_fid; m.S = d_S; m.area = d_area; m.rL = d_rL; m.rR = d_rR;
     m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
-    m.sS = d_sS; m.sR = d_sR; m.sN = d_sN; m.sA = d_sA; m.mate = d_mate; m.g_r = d_gr;
+    m.sSR = reinterpret_cast<const double2*>(d_sSR);
+    m.sNA = reinterpret_cast<const double2*>(d_sNA);
+    m.mate = d_mate; m.g_r = d_gr;
     m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr; m.tile_map = c->tile_map;
     DState& s = c->s;
     s.jac = nullptr;
@@ -597,7 +612,8 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
         (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.Wf, 5 * (NS + NG))) || (rc = dalloc(c, &s.dt, NR)) ||
         (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / kWG + 1))) ||
-        (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb))) {
+        (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb)) ||
+        (rc = dalloc(c, &c->d_mpart, 10 * (size_t)nb * c->mon_split))) {
         free_device(c);
         return rc;
     }
@@ -1029,8 +1045,9 @@ static int batch_iteration(e3d_ctx* c) {
     if ((rc = exchange(c, 0))) return rc;
     size_t ev = 0;
     ev_begin(c, KI_MONITORS, &ev);
-    launch_block_monitors(c->nb, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost, c->d_wall_face,
-                          c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mon, c->stream);
+    launch_block_monitors(c->nb, c->mon_split, c->d_tile0, c->d_ntile, c->d_nreal, c->d_wall_off, c->d_wall_ghost,
+                          c->d_wall_face, c->d_fcenter, c->m, c->s.W, c->s.part, c->d_blk_qcmac, c->d_mpart, c->d_mon,
+                          c->stream);
     launch_batch_reduce(c->n_inst, c->d_inst_b0, c->d_mon, c->d_prm, c->d_iter, c->d_rms0, c->d_active, c->d_frozen,
                         c->d_hist, c->hist_cap, c->d_nactive, c->d_clock + 1, c->stream);
     launch_batch_freeze(c->m, c->d_frozen, c->d_g_blk, c->s.W, c->s.Wn, c->stream);

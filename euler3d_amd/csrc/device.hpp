@@ -34,13 +34,12 @@ struct DMesh {
     const int* g_face;    // [NG] face of each ghost
     const int* g_kind;    // [NG] GK_* bits
     const int* wg_block;  // [NRp/kWG] block of each workgroup-sized tile
-    // Per-slot copies of face geometry for the fused path, slot-major so a
-    // workgroup streams them coalesced: component q of slot k of cell c at
-    // (q*nslot + k)*NRp + c.
-    const double* sS;     // [3][nslot][NRp] area vector S of the face, negated when this cell is its right cell
-    const double* sR;     // [3][nslot][NRp] r vector of the face on this cell's side
-    const double* sN;     // [3][nslot][NRp] unit normal S/|S|
-    const double* sA;     // [nslot][NRp] |S|
+    // Per-slot copies of face geometry for the fused path in 16-byte pairs,
+    // slot-major so a workgroup streams them coalesced: pair p of slot k of
+    // cell c at (p*nslot + k)*NRp + c.
+    const double2* sSR;   // [3][nslot][NRp] (Sx, Sy), (Sz, rx), (ry, rz): area vector S of the face, negated
+                          // when this cell is its right cell, and the face's r vector on this cell's side
+    const double2* sNA;   // [2][nslot][NRp] (nx, ny), (nz, |S|): unit normal S/|S| of the face as stored, |S|
     const int* mate;      // [nslot][NRp] face-state record of the other cell of the face; bit 31: this cell is right
     const double* g_r;    // [3][NG] r vector of each ghost on its face (the reflected right vector)
     // Instances: a context may hold a batch of independent meshes, each with its

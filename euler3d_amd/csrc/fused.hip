@@ -28,6 +28,31 @@ __device__ __forceinline__ double slotv(const double* a, int q, int nslot, int N
     return a[((size_t)q * nslot + k) * NRp + c];
 }
 
+// Element i >= 0 of an array whose base is the same for the whole wavefront,
+// addressed as base + 32-bit byte offset (i * sizeof(T) < 2^32, which the
+// upload's limit of 2^29 cells guarantees): one address register per lane
+// instead of two, and no 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, int i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)i * (unsigned)sizeof(T));
+}
+// Pair p of slot k of cell c in a per-slot pair array.
+__device__ __forceinline__ double2 pair32(const double2* a, int p, int nslot, int NRp, int k, int c) {
+    return ld32(a + ((size_t)p * nslot + k) * NRp, c);
+}
+// Unit normal and |S| of slot k of cell c.
+__device__ __forceinline__ double slot_normal(const DMesh& m, int k, int c, double n[3]) {
+    const double2 a = pair32(m.sNA, 0, m.nslot, m.NRp, k, c), b = pair32(m.sNA, 1, m.nslot, m.NRp, k, c);
+    n[0] = a.x;
+    n[1] = a.y;
+    n[2] = b.x;
+    return b.y;
+}
+__device__ __forceinline__ void load_w32(const double* W, int NA, int x, double w[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) w[v] = ld32(W + (size_t)v * NA, x);
+}
+
 // Face-state records in three planes so that a record is three loads: (rho, u)
 // and (v, w) as 16-byte pairs, p alone. Record r: pair q (q = 0, 1) at
 // Wf[2*q*nrec + 2*r], p at Wf[4*nrec + r]; the records of a wavefront's cells
@@ -78,8 +103,7 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
     const int gi = nb - NRp;
     const bool reflect = m.g_kind[gi] & GK_REFLECT;
     double n[3], rg[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
+    (void)slot_normal(m, k, c, n);  // S / area
     load3(m.g_r, m.NG, gi, rg);
     const size_t nrec = rec_count(m), rec = ghost_record(ns, NRp, gi);
 #pragma unroll
@@ -122,9 +146,16 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     double wc[5];
     load_w(s.W, m.NA, c, wc);
     const double vol = m.vol[c];
-    // pass 1: gradient and the neighbour extrema, faces in the reference's order
+    // pass 1: gradient and the neighbour extrema, faces in the reference's order.
+    // The neighbour ids of all slots are loaded first (slots k >= nf hold
+    // in-bounds padding and are not used), so each face then waits for one
+    // memory round trip (its neighbour's state and its geometry), not two.
+    __shared__ double r_lds[MAXS * 3][kWG];  // r vectors of the cell's faces, for passes 2 and 3
     double g[15], mx[5], mn[5];
     unsigned ghost_slots = 0;  // bit k: the neighbour across face slot k is a ghost
+    int nbv[MAXS];
+#pragma unroll
+    for (int k = 0; k < MAXS; ++k) nbv[k] = k < ns ? ld32(m.nbr + (size_t)k * NRp, c) : c;
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
 #pragma unroll
@@ -132,40 +163,46 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
-            double wn[5];
-            const int nb = m.nbr[(size_t)k * NRp + c];
-            if (nb >= NRp) ghost_slots |= 1u << k;
-            load_w(s.W, m.NA, nb, wn);
+            double wn[5], geo[3];
+            load_w32(s.W, m.NA, nbv[k], wn);
+            // S oriented away from this cell (Green-Gauss) or the least-squares
+            // weights; the r vector goes to LDS for the limiter and reconstruction
+            const double2 p1 = pair32(m.sSR, 1, ns, NRp, k, c), p2 = pair32(m.sSR, 2, ns, NRp, k, c);
+            if (GRAD == 0) {
+                const double2 p0 = pair32(m.sSR, 0, ns, NRp, k, c);
+                geo[0] = p0.x;
+                geo[1] = p0.y;
+                geo[2] = p1.x;
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) geo[d] = ld32(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
+            }
+            r_lds[k * 3 + 0][threadIdx.x] = p1.y;
+            r_lds[k * 3 + 1][threadIdx.x] = p2.x;
+            r_lds[k * 3 + 2][threadIdx.x] = p2.y;
+            if (nbv[k] >= NRp) ghost_slots |= 1u << k;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 if (wn[v] > mx[v]) mx[v] = wn[v];
                 if (wn[v] < mn[v]) mn[v] = wn[v];
             }
             if (GRAD == 0) {
-                // S oriented away from this cell (negated when the cell is the face's right cell)
-                double n[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) n[d] = slotv(m.sS, d, ns, NRp, k, c);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) g[v * 3 + d] += 0.5 * (wc[v] + wn[v]) * n[d];
+                    for (int v = 0; v < 5; ++v) g[v * 3 + d] += 0.5 * (wc[v] + wn[v]) * geo[d];
             } else {
-                double wt[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) wt[d] = m.lsqw[(size_t)(k * 3 + d) * NRp + c];
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) g[v * 3 + d] += wt[d] * (wn[v] - wc[v]);
+                    for (int d = 0; d < 3; ++d) g[v * 3 + d] += geo[d] * (wn[v] - wc[v]);
             }
         }
     if (GRAD == 0) {
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] /= vol;
     }
-    // pass 2: limiter over the faces; the r vectors are kept in LDS for pass 3
-    __shared__ double r_lds[MAXS * 3][kWG];
+    // pass 2: limiter over the faces, r vectors from LDS
     const double eps2 = k3 * (vol);
     double lim[5];
 #pragma unroll
@@ -179,10 +216,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
         if (k < nf) {
             double r[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                r[d] = slotv(m.sR, d, ns, NRp, k, c);
-                r_lds[k * 3 + d][threadIdx.x] = r[d];
-            }
+            for (int d = 0; d < 3; ++d) r[d] = r_lds[k * 3 + d][threadIdx.x];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double d2 = 0.0;
@@ -202,7 +236,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
             face_state(wc, lim, g, r, out);
             store_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, out);
             if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
-                ghost_face_state<VENKAT>(m, s, k, c, m.nbr[(size_t)k * NRp + c], g, wc, eps2, omega);
+                ghost_face_state<VENKAT>(m, s, k, c, nbv[k], g, wc, eps2, omega);
         }
 }
 
@@ -238,9 +272,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 const bool right = mt < 0;
                 const size_t rec = (size_t)(mt & 0x7fffffff);
                 double n[3], own[5], oth[5];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);
-                const double norm = m.sA[(size_t)k * NRp + c];
+                const double norm = slot_normal(m, k, c, n);
                 if (FIRST) sr += (fabs((wc[1]) * n[0] + (wc[2]) * n[1] + (wc[3]) * n[2]) + (c_center)) * norm;
                 if (ORDER2) {
                     load_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, own);
@@ -322,8 +354,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 double gw[5];
                 if (kind & GK_BC_MASK) {
                     double n[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) n[d] = slotv(m.sN, d, ns, NRp, k, c);  // S / area
+                    (void)slot_normal(m, k, c, n);  // S / area
                     if ((kind & GK_BC_MASK) == GK_SLIP) ghost_slip(wnew, n, gw);
                     else ghost_farfield(gas.gamma, wnew, fs, n, gw);
                 } else {

@@ -441,19 +441,41 @@ __global__ void __launch_bounds__(kWG) k_rms_partials(DMesh m, DState s, int nti
     }
 }
 
-// Per-block monitors: residual norms from the tile partials and the wall-pressure
-// force/moment sums (Convergence.cpp:27-144, AerodynamicParameters.cpp:163-239).
-// One workgroup per block; each block's wall faces are strided over the threads
-// and reduced in a fixed tree, so the result is deterministic.
-__global__ void __launch_bounds__(kWG) k_block_monitors(const int* blk_tile0, const int* blk_ntile, const int* blk_nreal,
-                                                        const int* wall_off, const int* wall_ghost, const int* wall_face,
-                                                        const double* fcenter, int NF, DMesh m, const double* W,
-                                                        const double* part, const double* blk_qcmac, double* out) {
-    const int b = blockIdx.x;
+// Sum over a workgroup of five per-thread values: shuffle tree within each
+// wavefront, then the wavefront sums in order. Valid in thread 0..4 (value v in
+// thread v) after the call.
+__device__ __forceinline__ double wg_sum5(const double a[5], double (*red)[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        double x = a[v];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < 5)
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
+    __syncthreads();
+    return t;
+}
+
+// Per-block monitors, first level: workgroup (b, p) of split workgroups per
+// block sums the block's wall-pressure forces and moments
+// (AerodynamicParameters.cpp:163-239) over wall faces p*kWG + t + j*split*kWG,
+// and the squared-residual tile partials of the block (Convergence.cpp:27-144)
+// over tiles in the same pattern; fixed trees, so the sums are deterministic.
+__global__ void __launch_bounds__(kWG) k_monitor_partials(int split, const int* blk_tile0, const int* blk_ntile,
+                                                          const int* wall_off, const int* wall_ghost,
+                                                          const int* wall_face, const double* fcenter, int NF, DMesh m,
+                                                          const double* W, const double* part,
+                                                          const double* blk_qcmac, double* mpart) {
+    const int b = blockIdx.x / split, p = blockIdx.x % split;
     const double qcmac = blk_qcmac[b];  // dynamic head x cmac of the block's instance
     __shared__ double red[kWG / 64][5];
     double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = wall_off[b] + threadIdx.x; q < wall_off[b + 1]; q += kWG) {
+    for (int q = wall_off[b] + p * kWG + threadIdx.x; q < wall_off[b + 1]; q += split * kWG) {
         const int gcell = wall_ghost[q], f = wall_face[q], in = m.g_in[gcell - m.NRp];
         const double ppbc = 0.5 * (W[(size_t)4 * m.NA + in] + W[(size_t)4 * m.NA + gcell]);
         const double cpbc = (ppbc - 1.) / qcmac;
@@ -465,40 +487,25 @@ __global__ void __launch_bounds__(kWG) k_block_monitors(const int* blk_tile0, co
         a[3] += z * fx - x * fz;
         a[4] += x * fy - y * fx;
     }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        double x = a[v];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 5) {
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
-        out[(size_t)b * 16 + 5 + threadIdx.x] = t;
-    }
-    // residual norms: tile partials strided over the threads, then a fixed tree
+    double t = wg_sum5(a, red);
+    if (threadIdx.x < 5) mpart[((size_t)b * split + p) * 10 + 5 + threadIdx.x] = t;
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = threadIdx.x; q < blk_ntile[b]; q += kWG)
+    for (int q = p * kWG + threadIdx.x; q < blk_ntile[b]; q += split * kWG)
 #pragma unroll
         for (int v = 0; v < 5; ++v) r[v] += part[(size_t)(blk_tile0[b] + q) * 8 + v];
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        double x = r[v];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 5) {
-        double rs = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWG / 64; ++w) rs += red[w][threadIdx.x];
-        out[(size_t)b * 16 + threadIdx.x] = (1. / blk_nreal[b]) * sqrt(rs);
-    }
+    t = wg_sum5(r, red);
+    if (threadIdx.x < 5) mpart[((size_t)b * split + p) * 10 + threadIdx.x] = t;
+}
+
+// Second level: per block, the split partials in order; residual norm
+// (1/N) sqrt(sum) (Convergence.cpp:35-42) and the force and moment sums.
+__global__ void __launch_bounds__(64) k_block_monitors(int nb, int split, const int* blk_nreal, const double* mpart,
+                                                       double* out) {
+    const int b = blockIdx.x * 6 + threadIdx.x / 10, q = threadIdx.x % 10;
+    if (threadIdx.x >= 60 || b >= nb) return;
+    double t = 0.0;
+    for (int p = 0; p < split; ++p) t += mpart[((size_t)b * split + p) * 10 + q];
+    out[(size_t)b * 16 + q] = q < 5 ? (1. / blk_nreal[b]) * sqrt(t) : t;
 }
 
 // ------------------------------------------------------------ host launchers
@@ -563,11 +570,13 @@ void launch_rms_partials(const DMesh& m, const DState& s, hipStream_t st) {
     const int nt = m.NRp / kWG;
     hipLaunchKernelGGL(k_rms_partials, dim3(nt), dim3(kWG), 0, st, m, s, nt);
 }
-void launch_block_monitors(int nb, const int* tile0, const int* ntile, const int* nreal, const int* wall_off,
-                           const int* wall_ghost, const int* wall_face, const double* fcenter, const DMesh& m,
-                           const double* W, const double* part, const double* blk_qcmac, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_block_monitors, dim3(nb), dim3(kWG), 0, st, tile0, ntile, nreal, wall_off, wall_ghost,
-                       wall_face, fcenter, m.NF, m, W, part, blk_qcmac, out);
+void launch_block_monitors(int nb, int split, const int* tile0, const int* ntile, const int* nreal,
+                           const int* wall_off, const int* wall_ghost, const int* wall_face, const double* fcenter,
+                           const DMesh& m, const double* W, const double* part, const double* blk_qcmac,
+                           double* mpart, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_monitor_partials, dim3(nb * split), dim3(kWG), 0, st, split, tile0, ntile, wall_off,
+                       wall_ghost, wall_face, fcenter, m.NF, m, W, part, blk_qcmac, mpart);
+    hipLaunchKernelGGL(k_block_monitors, dim3((nb + 5) / 6), dim3(64), 0, st, nb, split, nreal, mpart, out);
 }
 
 }  // namespace e3d
