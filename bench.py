@@ -38,7 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_kernels_bump192x64x64_traffic.json")
+# HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) for the two
+# benchmark workloads: (nx, ny, nz, parts, scheme, brick, order) -> summary file.
+TRAFFIC_FILES = {
+    (192, 64, 64, (2, 2, 2), "roe-gg-vk", 4, "reference"): "fused_kernels_bump192x64x64_traffic.json",
+    (384, 128, 128, (1, 1, 1), "ausm-lsq-bj", 4, "reference"): "fused_kernels_bump384x128x128_ausm_lsq_bj_traffic.json",
+}
 
 # Schemes of the benchmark workloads: 4.7M-cell channel in 8 blocks (Roe,
 # Green-Gauss, Venkatakrishnan K=5, M=0.675) and the 37.7M-cell single block
@@ -178,9 +183,9 @@ def main():
         ms, n = ctx.kernel_time(name)
         kt[name] = {"avg_ms": ms, "launches": n}
     measured = {}
-    if os.path.exists(TRAFFIC_FILE) and (a.nx, a.ny, a.nz, tuple(a.parts), a.scheme, a.brick, a.order) == \
-            (192, 64, 64, (2, 2, 2), "roe-gg-vk", 4, "reference"):
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILES.get((a.nx, a.ny, a.nz, tuple(a.parts), a.scheme, a.brick, a.order))
+    if tfile and os.path.exists(os.path.join(ROOT, "profiles", tfile)):
+        with open(os.path.join(ROOT, "profiles", tfile)) as f:
             measured = json.load(f)["hbm_bytes_per_launch"]
     rooflines = {}
     for name in ("interp", "flux_update"):

@@ -450,8 +450,8 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     c->face_pos.assign(nb, {});
     if (c->ordering != ORDER_REFERENCE)
         for (int b = 0; b < nb; ++b) block_layout(*src[b].B, c->ordering, c->cell_pos[b], c->face_pos[b]);
-    if (NRp + NG >= (1L << 29) || NF >= (1L << 31) - 1)  // kernels address cell arrays with 32-bit byte offsets
-        return set_error(c, E3D_EINVAL, "mesh too large: at most 2^29 cells including ghosts");
+    if (NRp + NG >= (1L << 28) || NF >= (1L << 31) - 1)  // kernels address cell arrays with 32-bit byte offsets
+        return set_error(c, E3D_EINVAL, "mesh too large: at most 2^28 cells including ghosts");
     DMesh& m = c->m;
     m.NRp = (int)NRp;
     m.NG = (int)NG;

@@ -29,12 +29,20 @@ __device__ __forceinline__ double slotv(const double* a, int q, int nslot, int N
 }
 
 // Element i >= 0 of an array whose base is the same for the whole wavefront,
-// addressed as base + 32-bit byte offset (i * sizeof(T) < 2^32, which the
-// upload's limit of 2^29 cells guarantees): one address register per lane
-// instead of two, and no 64-bit address arithmetic.
+// addressed as base + 32-bit byte offset (i * sizeof(T) < 2^32 for T up to 16
+// bytes, which the upload's limit of 2^28 cells guarantees): one address
+// register per lane instead of two, and no 64-bit address arithmetic.
 template <class T>
 __device__ __forceinline__ T ld32(const T* base, int i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, int i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)i * (unsigned)sizeof(T)) = v;
+}
+__device__ __forceinline__ void store_w32(double* W, int NA, int x, const double w[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) st32(W + (size_t)v * NA, x, w[v]);
 }
 // Pair p of slot k of cell c in a per-slot pair array.
 __device__ __forceinline__ double2 pair32(const double2* a, int p, int nslot, int NRp, int k, int c) {
@@ -78,6 +86,25 @@ __device__ __forceinline__ void store_rec(double* Wf, size_t nrec, size_t rec, c
     reinterpret_cast<double2*>(Wf)[rec] = make_double2(w[0], w[1]);
     reinterpret_cast<double2*>(Wf + 2 * nrec)[rec] = make_double2(w[2], w[3]);
     Wf[4 * nrec + rec] = w[4];
+}
+
+// Record of cell c on its face slot k (record k*NRp + c), with the slot's
+// plane base uniform and a 32-bit offset per lane.
+__device__ __forceinline__ void load_own_rec(const double* Wf, size_t nrec, int NRp, int k, int c, double w[5]) {
+    const size_t r0 = (size_t)k * NRp;
+    const double2 a = ld32(reinterpret_cast<const double2*>(Wf) + r0, c);
+    const double2 b = ld32(reinterpret_cast<const double2*>(Wf + 2 * nrec) + r0, c);
+    w[0] = a.x;
+    w[1] = a.y;
+    w[2] = b.x;
+    w[3] = b.y;
+    w[4] = ld32(Wf + 4 * nrec + r0, c);
+}
+__device__ __forceinline__ void store_own_rec(double* Wf, size_t nrec, int NRp, int k, int c, const double w[5]) {
+    const size_t r0 = (size_t)k * NRp;
+    st32(reinterpret_cast<double2*>(Wf) + r0, c, make_double2(w[0], w[1]));
+    st32(reinterpret_cast<double2*>(Wf + 2 * nrec) + r0, c, make_double2(w[2], w[3]));
+    st32(Wf + 4 * nrec + r0, c, w[4]);
 }
 
 // W + (lim*grad).r, accumulated per component as FluxScheme.cpp:127-134 does.
@@ -140,12 +167,12 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     const int tile = tile_of(m, ntiles);
     if (tile >= ntiles || !instance_active(m, tile_instance(m, tile))) return;
     const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
+    const int nf = ld32(m.nf, c);
     if (nf == 0) return;
     const int NRp = m.NRp, ns = m.nslot;
     double wc[5];
-    load_w(s.W, m.NA, c, wc);
-    const double vol = m.vol[c];
+    load_w32(s.W, m.NA, c, wc);
+    const double vol = ld32(m.vol, c);
     // pass 1: gradient and the neighbour extrema, faces in the reference's order.
     // The neighbour ids of all slots are loaded first (slots k >= nf hold
     // in-bounds padding and are not used), so each face then waits for one
@@ -234,7 +261,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 #pragma unroll
             for (int d = 0; d < 3; ++d) r[d] = r_lds[k * 3 + d][threadIdx.x];
             face_state(wc, lim, g, r, out);
-            store_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, out);
+            store_own_rec(s.Wf, rec_count(m), NRp, k, c, out);
             if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
                 ghost_face_state<VENKAT>(m, s, k, c, nbv[k], g, wc, eps2, omega);
         }
@@ -253,29 +280,34 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
     if (!instance_active(m, inst)) return;
     const double* fs = m.inst_fs + 5 * inst;
     const int c = tile * kWG + threadIdx.x;
-    const int nf = m.nf[c];
+    const int nf = ld32(m.nf, c);
     const int NRp = m.NRp, ns = m.nslot;
     const size_t ghost0 = (size_t)ns * NRp;
     double res2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (nf != 0) {
         double cv[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ds[5], wc[5];
-        if (!ORDER2 || FIRST) load_w(s.W, m.NA, c, wc);
+        if (!ORDER2 || FIRST) load_w32(s.W, m.NA, c, wc);
         double c_center = 0.0, sr = 0.0;
         if (FIRST) c_center = sqrt(gas.gamma * wc[4] / wc[0]);
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-            ds[v] = DISS ? s.diss[(size_t)v * NRp + c] * (1.0 - beta) : s.diss[(size_t)v * NRp + c];
+            ds[v] = DISS ? ld32(s.diss + (size_t)v * NRp, c) * (1.0 - beta) : ld32(s.diss + (size_t)v * NRp, c);
+        // the other side's record index of every slot first (padding slots
+        // hold 0, in bounds), so each face then waits for one round trip
+        int mtv[MAXS];
+#pragma unroll
+        for (int k = 0; k < MAXS; ++k) mtv[k] = k < ns ? ld32(m.mate + (size_t)k * NRp, c) : 0;
 #pragma unroll
         for (int k = 0; k < MAXS; ++k)
             if (k < nf) {
-                const int mt = m.mate[(size_t)k * NRp + c];
+                const int mt = mtv[k];
                 const bool right = mt < 0;
                 const size_t rec = (size_t)(mt & 0x7fffffff);
                 double n[3], own[5], oth[5];
                 const double norm = slot_normal(m, k, c, n);
                 if (FIRST) sr += (fabs((wc[1]) * n[0] + (wc[2]) * n[1] + (wc[3]) * n[2]) + (c_center)) * norm;
                 if (ORDER2) {
-                    load_rec(s.Wf, rec_count(m), (size_t)k * NRp + c, own);
+                    load_own_rec(s.Wf, rec_count(m), NRp, k, c, own);
                     load_rec(s.Wf, rec_count(m), rec, oth);
                 } else {
 #pragma unroll
@@ -309,14 +341,14 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             }
         if (DISS) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s.diss[(size_t)v * NRp + c] = ds[v];
+            for (int v = 0; v < 5; ++v) st32(s.diss + (size_t)v * NRp, c, ds[v]);
         }
         if (LAST) {  // the last stage's convective residual feeds the monitors and the array getters
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s.conv[(size_t)v * NRp + c] = cv[v];
+            for (int v = 0; v < 5; ++v) st32(s.conv + (size_t)v * NRp, c, cv[v]);
         }
         double w0[5], wnew[5], dt;
-        const double vol = m.vol[c];
+        const double vol = ld32(m.vol, c);
         if (FIRST) {
             w0[0] = wc[0];
             w0[1] = wc[0] * wc[1];
@@ -325,16 +357,15 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             w0[4] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
             dt = cfl * vol / sr;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s.W0[(size_t)v * NRp + c] = w0[v];
-            s.dt[c] = dt;
+            for (int v = 0; v < 5; ++v) st32(s.W0 + (size_t)v * NRp, c, w0[v]);
+            st32(s.dt, c, dt);
         } else {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) w0[v] = s.W0[(size_t)v * NRp + c];
-            dt = s.dt[c];
+            for (int v = 0; v < 5; ++v) w0[v] = ld32(s.W0 + (size_t)v * NRp, c);
+            dt = ld32(s.dt, c);
         }
         rk_update(gas.gm1, alpha * dt / vol, w0, cv, ds, wnew);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + c] = wnew[v];
+        store_w32(s.Wn, m.NA, c, wnew);
         if (LAST) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -346,7 +377,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
 #pragma unroll
         for (int k = 0; k < MAXS; ++k)
             if (k < nf) {
-                const size_t rec = (size_t)(m.mate[(size_t)k * NRp + c] & 0x7fffffff);
+                const size_t rec = (size_t)(mtv[k] & 0x7fffffff);
                 if (rec < ghost0) continue;
                 const int gi = (int)(rec - ghost0);
                 const int gcell = NRp + gi;
