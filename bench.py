@@ -53,10 +53,12 @@ SCHEMES = {
                       label="2nd-order Roe + Green-Gauss + Venkatakrishnan K=5, M=0.675"),
     "ausm-lsq-bj": dict(mach=0.7, aoa=3.0, flux=1, gradient=1, limiter=0, k=5.0, omega=1e-16,
                         label="2nd-order AUSM + least squares + Barth-Jespersen omega=1e-16, M=0.7, 3 deg"),
+    "roe-1st": dict(mach=0.5, aoa=0.0, flux=0, gradient=0, limiter=1, k=5.0, omega=1e-16, order=1,
+                    label="1st-order Roe, M=0.5"),
 }
 
 
-def algorithmic_bytes(blocks, lsq=False):
+def algorithmic_bytes(blocks, lsq=False, order=2):
     """Algorithmic bytes per launch of the two stage kernels, and per cell-stage of the whole step.
 
     The compulsory-traffic model of this path: every array element counted once
@@ -67,13 +69,17 @@ def algorithmic_bytes(blocks, lsq=False):
     cell-to-cell and cell-to-face lists 16 B each per cell; faces only for r);
     flux_update does the residual kernel K_B = 200(N+G) + 80F + 120N and the
     update + BC kernel K_C = 176N + 112G of that model in one launch. Once per
-    iteration: saveW0, spectral radius, time step and monitors, 232N + 24F."""
+    iteration: saveW0, spectral radius, time step and monitors, 232N + 24F.
+    First order has no interp kernel and its flux kernel reads W of both sides
+    instead of face states: 40(N+G) + 32F + 120N, plus K_C."""
     N = sum(b["nreal"] for b in blocks)
     G = sum(b["nall"] - b["nreal"] for b in blocks)
     F = sum(b["nface"] for b in blocks)
     ka = 200 * (N + G) + 128 * N + 56 * F + 24 * G if lsq else 208 * (N + G) + 80 * F + 16 * N
-    kb = 200 * (N + G) + 80 * F + 120 * N
+    kb = 200 * (N + G) + 80 * F + 120 * N if order == 2 else 40 * (N + G) + 32 * F + 120 * N
     kc = 176 * N + 112 * G
+    if order != 2:
+        ka = 0
     per_cell_stage = (ka + kb + kc + (232 * N + 24 * F) / 5) / N
     return {"interp": ka, "flux_update": kb + kc}, per_cell_stage, N, G, F
 
@@ -92,7 +98,8 @@ def cpu_baselines(a, sc, blocks):
         m = E.Mesh.bump(96, 32, 32, 0.1, 4, (2, 2, 2))
         blocks = [m.block(b) for b in range(m.nblocks)]
         sample = "the same scheme on the 96x32x32x6 channel (589,824 cells in 8 blocks)"
-    prm = O.params(sc["mach"], sc["aoa"], flux=sc["flux"], order=2, gradient=sc["gradient"], limiter=sc["limiter"],
+    prm = O.params(sc["mach"], sc["aoa"], flux=sc["flux"], order=sc.get("order", 2), gradient=sc["gradient"],
+                   limiter=sc["limiter"],
                    k=sc["k"], omega=sc["omega"], stages=5, max_iter=10 ** 9, criterion=-1e9)
     threads = max(1, min(len(blocks), os.cpu_count() or 1, 16))
     S = O.Solver(blocks, prm)
@@ -136,6 +143,9 @@ def main():
                          "or < 0 for tets in a pseudo-random order seeded by -brick")
     ap.add_argument("--order", choices=["reference", "rcm"], default="reference",
                     help="device memory order of cells and faces (results are the same bit for bit)")
+    ap.add_argument("--graph", action="store_true",
+                    help="advance through the batch API with one instance: stop test on the device and a "
+                         "hipGraph of two iterations replayed (no per-kernel timing)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -155,24 +165,35 @@ def main():
     mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, a.brick, tuple(a.parts))
     blocks = [mesh.block(b) for b in range(mesh.nblocks)]
     t_mesh = time.perf_counter() - t_mesh
-    alg, model_bytes, N, G, F = algorithmic_bytes(blocks, lsq=sc["gradient"] == 1)
-    params = E.make_params(sc["mach"], sc["aoa"], flux=sc["flux"], order=2, gradient=sc["gradient"],
+    order = sc.get("order", 2)
+    alg, model_bytes, N, G, F = algorithmic_bytes(blocks, lsq=sc["gradient"] == 1, order=order)
+    params = E.make_params(sc["mach"], sc["aoa"], flux=sc["flux"], order=order, gradient=sc["gradient"],
                            limiter=sc["limiter"], k=sc["k"], omega=sc["omega"], stages=5, max_iter=10 ** 9,
                            criterion=-1e9)
     ctx = E.Context(0)
     ctx.set_params(params)
     ctx.set_ordering(E.ORDER_RCM if a.order == "rcm" else E.ORDER_REFERENCE)
     print(f"[bench] mesh built in {t_mesh:.1f}s; uploading", file=sys.stderr, flush=True)
-    ctx.upload(mesh)
+    if a.graph:
+        ctx.upload_batch([mesh], [dict(mach=sc["mach"], aoa_deg=sc["aoa"], criterion=-1e9, max_iter=10 ** 9)])
+    else:
+        ctx.upload(mesh)
     ctx.init_freestream()
     print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s); "
           f"warmup {a.warmup}", file=sys.stderr, flush=True)
     if a.warmup:
-        ctx.iterate(a.warmup, history=False)
+        if a.graph:
+            ctx.batch_run(a.warmup, use_graph=True)
+        else:
+            ctx.iterate(a.warmup, history=False)
     ctx.op("sync")
     E.lib().e3d_enable_timing(ctx._h, 1)
     t0 = time.perf_counter()
-    hist = ctx.iterate(a.steps)
+    if a.graph:
+        ctx.batch_run(a.steps, use_graph=True)
+        hist = ctx.batch_history(0, a.warmup + a.steps)[-1:]
+    else:
+        hist = ctx.iterate(a.steps)
     ctx.op("sync")
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -190,12 +211,14 @@ def main():
     rooflines = {}
     for name in ("interp", "flux_update"):
         t = kt[name]
-        ach = alg[name] / (t["avg_ms"] * 1e-3) / 1e9 if t["avg_ms"] > 0 else 0.0
+        if t["avg_ms"] <= 0 or alg[name] == 0:  # not launched, or launches inside a graph (not timed)
+            continue
+        ach = alg[name] / (t["avg_ms"] * 1e-3) / 1e9
         rooflines[name] = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": measured.get(name),
                            "algorithmic_bytes_per_launch": alg[name], "avg_ms": t["avg_ms"],
                            "share_of_step": t["avg_ms"] * t["launches"] / (wall * 1e3)}
-    dominant = max(rooflines, key=lambda k: rooflines[k]["share_of_step"])
+    dominant = max(rooflines, key=lambda k: rooflines[k]["share_of_step"]) if rooflines else None
     rms_ro = hist[-1]["rms"][0] if hist else float("nan")
     print(f"[bench] {a.steps} steps in {wall:.3f}s; rms_ro {rms_ro:.6e}; kernels "
           + ", ".join(f"{k} {v['avg_ms']:.3f}ms x{v['launches']}" for k, v in kt.items()), file=sys.stderr, flush=True)
@@ -225,10 +248,10 @@ def main():
                                f"{mesh.nblocks} block(s) on one device, {sc['label']}, 5-stage RK, CFL 1.5",
                    "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages,
                    "cell_numbering": "scrambled" if a.brick < 0 else f"brick {a.brick}",
-                   "memory_order": a.order},
+                   "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate"},
         "model_bytes_per_cell_stage": model_bytes,
         "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / model_bytes / 1e6),
-        "roofline": rooflines[dominant],
+        "roofline": rooflines[dominant] if dominant else None,
         "rooflines": rooflines,
         "kernels": kt,
         "cpu_baseline": cpu,
