@@ -104,7 +104,8 @@ def scheme_params(order, flux="roe", gradient="gg", limiter="vk", k=5.0, omega=1
                 limiter=LIM_BARTH_JESPERSEN if limiter == "bj" else LIM_VENKATAKRISHNAN, k=k, omega=omega, stages=5)
 
 
-def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=None, use_graph=True):
+def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=None, use_graph=True, chunk=2000,
+                log=sys.stderr, history_every=0):
     """Runs one batch; warm: per instance [5 x nall] initial states (None: freestream).
     Returns per-instance results and the final real-cell states."""
     ctx = Context(0)
@@ -123,65 +124,147 @@ def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=N
             ctx.set(b, ARR_W, Wi)
     ctx.op("sync")
     t0 = time.perf_counter()
-    done, status = ctx.batch_run(max_iter, use_graph=use_graph)
+    done = 0
+    while done < max_iter:  # in chunks, reporting progress after each
+        n, status = ctx.batch_run(min(chunk, max_iter - done), use_graph=use_graph)
+        done += n
+        active = sum(1 for x in status if not x["stopped"])
+        if log is not None:
+            print(f"[warmstart]   {len(conds)} instances: {done} iterations, {active} still running "
+                  f"({time.perf_counter() - t0:.1f}s)", file=log, flush=True)
+        if active == 0 or n == 0:
+            break
     wall = time.perf_counter() - t0
     out = []
     states = []
     for i, (c, s) in enumerate(zip(conds, status)):
         (b,) = ctx.batch_blocks(i)
         first = ctx.batch_history(i, 1)
+        if history_every:
+            h = ctx.batch_history(i, s["iterations"])
+            hist_rows = [[r["iter"], r["drop"][0], r["cl"], r["cd"]] for r in h[::history_every]]
+        else:
+            hist_rows = None
         out.append(dict(c, iterations=s["iterations"], stop_seconds=s["stop_seconds"], drop=s["last"]["drop"][0],
                         cl=s["last"]["cl"], cd=s["last"]["cd"], rms0=first[0]["rms"] if first else None,
                         drop0=first[0]["drop"][0] if first else None,
-                        converged=bool(s["last"]["drop"][0] <= criterion)))
+                        converged=bool(s["last"]["drop"][0] <= criterion), history=hist_rows))
         states.append(ctx.get(b, ARR_W))
     ctx.close()
     return out, states, wall
 
 
-def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000, criterion=-8.0, steps=3000,
-        batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, log=sys.stderr):
-    scheme = scheme_params(order)
-    train_c = conditions(range(n_train))
-    held_c = conditions(range(5000, 5000 + n_heldout))
+def predict_chunked(net, x, chunk=1 << 22):
+    return np.concatenate([net.predict(x[i:i + chunk]) for i in range(0, len(x), chunk)]) if len(x) else \
+        np.zeros((0, 5), np.float32)
 
-    def meshes_for(conds):
-        return [Mesh.bump(nx, ny, nz, c["h"], 4, (1, 1, 1)) for c in conds]
 
-    # 1. training fields from the batched solver
-    t0 = time.perf_counter()
-    tm = meshes_for(train_c)
-    res, states, t_gen = solve_batch(tm, train_c, scheme, max_iter, criterion)
+def bump_meshes(conds, nx, ny, nz):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(8) as pool:  # the builder runs in C++ without the GIL
+        return list(pool.map(lambda c: Mesh.bump(nx, ny, nz, c["h"], 4, (1, 1, 1)), conds))
+
+
+def training_set(conds, nx, ny, nz, scheme, max_iter, criterion, log=sys.stderr):
+    """Fields from the batched solver for every condition (freestream start, up to
+    the criterion or max_iter), as network inputs and targets of their real cells."""
+    tm = bump_meshes(conds, nx, ny, nz)
+    res, states, t_gen = solve_batch(tm, conds, scheme, max_iter, criterion, log=log)
     X, T = [], []
-    for m, c, W in zip(tm, train_c, states):
+    for m, c, W in zip(tm, conds, states):
         blk = m.block(0)
         if not np.isfinite(W).all():
             continue  # a diverged case is no training data
         X.append(features(blk, c)[:blk["nreal"]])
         T.append(targets(W, c, blk["nreal"]))
     X, T = np.concatenate(X), np.concatenate(T)
-    print(f"[warmstart] {len(train_c)} training fields ({sum(r['converged'] for r in res)} converged) in "
+    print(f"[warmstart] {len(conds)} training fields ({sum(r['converged'] for r in res)} at {criterion:g}) in "
           f"{t_gen:.1f}s; {len(X)} samples", file=log, flush=True)
+    return X, T, res, t_gen
+
+
+class Surrogate:
+    """Networks trained in stages on the GPU: the first fits the fields; each
+    further one fits what the ones before it leave (rescaled to unit size), so
+    the smooth part of the error, which the solver removes slowly, shrinks."""
+
+    def __init__(self, X, T, stages=1, steps=30000, batch=8192, lr=2e-3, width=256, depth=5, log=sys.stderr):
+        self.nets, self.scales, self.loss = [], [], None
+        t1 = time.perf_counter()
+        R = T
+        for stage in range(stages):
+            sc = np.ones(5, np.float32) if stage == 0 else R.std(axis=0).astype(np.float32) + 1e-12
+            net = MLP(N_FEATURES, 5, width, depth, seed=1 + 10 * stage)
+            net.set_data(X, R / sc)
+            ls = net.train(steps, batch=batch, lr=lr, seed=2 + 10 * stage)
+            self.loss = ls if self.loss is None else self.loss
+            self.nets.append(net)
+            self.scales.append(sc)
+            if stage < stages - 1:
+                R = R - predict_chunked(net, X) * sc
+                print(f"[warmstart] stage {stage}: loss {ls[0]:.3e} -> {np.mean(ls[-50:]):.3e}; residual targets "
+                      f"std {R.std(axis=0)}", file=log, flush=True)
+        self.train_seconds = time.perf_counter() - t1
+        self.step_ms, flops = self.nets[0].time_step(batch, reps=5)
+        self.tflops = flops / self.step_ms / 1e9
+        # fit on the training fields themselves (the generalisation gap is this vs the held-out error)
+        sub = np.random.default_rng(3).choice(len(X), size=min(len(X), 1 << 20), replace=False)
+        err = np.abs(self(X[sub]) - T[sub]) * TARGET_SCALE
+        self.train_max_err, self.train_mean_err = float(err.max()), float(err.mean())
+        print(f"[warmstart] trained {stages} network(s) x {steps} steps in {self.train_seconds:.1f}s, loss "
+              f"{self.loss[0]:.3e} -> {np.mean(self.loss[-50:]):.3e}; step {self.step_ms:.3f} ms = "
+              f"{self.tflops:.1f} TFLOP/s; error on training cells: max {self.train_max_err:.4f}, "
+              f"mean {self.train_mean_err:.2e}", file=log, flush=True)
+
+    def __call__(self, x, n=None):
+        """Prediction of the first n networks (all by default)."""
+        k = len(self.nets) if n is None else n
+        return sum(predict_chunked(net, x) * sc for net, sc in zip(self.nets[:k], self.scales[:k]))
+
+    def state(self, block, cond, n=None):
+        """Initial primitive state [5 x nall] of a block for a condition."""
+        return state_from_prediction(self(features(block, cond), n), cond)
+
+    def close(self):
+        for net in self.nets:
+            net.close()
+        self.nets = []
+
+
+def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000, criterion=-8.0, steps=3000,
+        batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, train_criterion=None, boost=0, log=sys.stderr):
+    scheme = scheme_params(order)
+    train_c = conditions(range(n_train))
+    held_c = conditions(range(5000, 5000 + n_heldout))
+    t0 = time.perf_counter()
+    # 1. training fields from the batched solver; they need only be converged
+    # well below the network's own error
+    tcrit = criterion if train_criterion is None else train_criterion
+    X, T, res, t_gen = training_set(train_c, nx, ny, nz, scheme, max_iter, tcrit, log)
     # 2. training on the GPU
-    net = MLP(N_FEATURES, 5, width, depth, seed=1)
-    net.set_data(X, T)
-    t1 = time.perf_counter()
-    loss = net.train(steps, batch=batch, lr=lr, seed=2)
-    t_train = time.perf_counter() - t1
-    ms, flops = net.time_step(batch, reps=5)
-    print(f"[warmstart] trained {steps} steps in {t_train:.1f}s, loss {loss[0]:.3e} -> {np.mean(loss[-50:]):.3e}; "
-          f"step {ms:.3f} ms = {flops / ms / 1e9:.1f} TFLOP/s", file=log, flush=True)
+    sur = Surrogate(X, T, 1 + boost, steps, batch, lr, width, depth, log)
+    loss, t_train, ms = sur.loss, sur.train_seconds, sur.step_ms
+    flops = sur.tflops * ms * 1e9
+    train_max_err, train_mean_err = sur.train_max_err, sur.train_mean_err
     # 3. held-out: cold, then warm from the network with the cold iteration-0 norms as reference
-    hm = meshes_for(held_c)
-    cold, cold_states, t_cold = solve_batch(hm, held_c, scheme, max_iter, criterion)
+    hm = bump_meshes(held_c, nx, ny, nz)
+    cold, cold_states, t_cold = solve_batch(hm, held_c, scheme, max_iter, criterion, history_every=50)
     t2 = time.perf_counter()
-    warm_states = []
-    for m, c in zip(hm, held_c):
-        blk = m.block(0)
-        warm_states.append(state_from_prediction(net.predict(features(blk, c)), c))
+    warm_states = [sur.state(m.block(0), c) for m, c in zip(hm, held_c)]
     t_infer = time.perf_counter() - t2
+    # the surrogates with fewer stages, for comparison (iterations to the criterion only)
+    by_stages = []
+    for k in range(1, boost + 1):
+        ws_k = [sur.state(m.block(0), c, k) for m, c in zip(hm, held_c)]
+        wk, _, _ = solve_batch(hm, held_c, scheme, max_iter, criterion, warm=ws_k, rms_ref=[r["rms0"] for r in cold])
+        ok = [(a, b) for a, b in zip(cold, wk) if a["converged"] and b["converged"]]
+        by_stages.append(dict(networks=k, converged_both=len(ok),
+                              iterations_ratio=sum(a["iterations"] for a, _ in ok) / max(1, sum(b["iterations"] for _, b in ok)),
+                              mean_initial_drop=float(np.mean([b["drop0"] for b in wk]))))
+        print(f"[warmstart] {k} network(s): cold/warm iterations {by_stages[-1]['iterations_ratio']:.3f}x, mean "
+              f"initial drop {by_stages[-1]['mean_initial_drop']:.2f}", file=log, flush=True)
     warm, _, t_warm = solve_batch(hm, held_c, scheme, max_iter, criterion, warm=warm_states,
-                                  rms_ref=[r["rms0"] for r in cold])
+                                  rms_ref=[r["rms0"] for r in cold], history_every=50)
     # converged coefficients: both starts taken `deeper` decades further, where Cl and Cd no longer move
     deep_cold = deep_warm = None
     if deeper > 0:
@@ -192,13 +275,16 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
     for c, w, cs, ws, m in zip(cold, warm, cold_states, warm_states, hm):
         nr = m.block(0)["nreal"]
         field_err = float(np.max(np.abs(ws[:, :nr] - cs[:, :nr])))  # surrogate vs converged cold state, real cells
+        field_mean_err = float(np.mean(np.abs(ws[:, :nr] - cs[:, :nr])))
         rows.append(dict(seed=c["seed"], mach=c["mach"], aoa_deg=c["aoa_deg"], h=c["h"], field_max_err=field_err,
+                         field_mean_err=field_mean_err,
                          cold_iterations=c["iterations"], warm_iterations=w["iterations"],
                          cold_seconds=c["stop_seconds"], warm_seconds=w["stop_seconds"] + t_infer / len(held_c),
                          cold_converged=c["converged"], warm_converged=w["converged"],
                          warm_initial_drop=w["drop0"], cl_cold=c["cl"], cl_warm=w["cl"], cd_cold=c["cd"], cd_warm=w["cd"],
                          cl_rel_diff=abs(w["cl"] - c["cl"]) / max(abs(c["cl"]), 1e-300),
-                         cd_rel_diff=abs(w["cd"] - c["cd"]) / max(abs(c["cd"]), 1e-300)))
+                         cd_rel_diff=abs(w["cd"] - c["cd"]) / max(abs(c["cd"]), 1e-300),
+                         cold_history=c["history"], warm_history=w["history"]))
     if deeper > 0:
         for r, dc, dw in zip(rows, deep_cold, deep_warm):
             r.update(deep_converged=bool(dc["converged"] and dw["converged"]), cl_deep_cold=dc["cl"],
@@ -207,6 +293,8 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
                      cd_deep_rel_diff=abs(dw["cd"] - dc["cd"]) / max(abs(dc["cd"]), 1e-300))
     both = [r for r in rows if r["cold_converged"] and r["warm_converged"]]
     summary = dict(n_train=n_train, n_heldout=n_heldout, mesh=[nx, ny, nz], order=order, criterion=criterion,
+                   train_criterion=tcrit, width=width, boost=boost, by_stages=by_stages, train_max_err=train_max_err,
+                   train_mean_err=train_mean_err,
                    max_iter=max_iter, samples=int(len(X)), train_steps=steps, batch=batch,
                    train_seconds=t_train, train_step_ms=ms, train_tflops=flops / ms / 1e9,
                    loss_first=float(loss[0]), loss_last=float(np.mean(loss[-50:])), generate_seconds=t_gen,
@@ -222,8 +310,55 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
                    max_cd_deep_rel_diff=max([r["cd_deep_rel_diff"] for r in rows if r.get("deep_converged")],
                                             default=None),
                    heldout=rows, wall_seconds=time.perf_counter() - t0)
-    net.close()
+    sur.close()
     return summary
+
+
+def run_large(n_train=256, nx_train=48, train_iters=8000, steps=30000, stages=3, width=512, nx=384, mach=0.7,
+              aoa_deg=3.0, h=0.1, iters=3000, log=sys.stderr):
+    """The warm start on one large mesh it has never seen: the surrogate (it
+    takes coordinates, not mesh indices) is trained on fields of small channels
+    (nx_train x nx_train/3 x nx_train/3 hexes) solved with the same scheme
+    (2nd-order AUSM, least squares, Barth-Jespersen) for train_iters
+    iterations, then predicts the initial field of the nx x nx/3 x nx/3 channel.
+    Cold and warm runs of `iters` iterations each; the drop of both is measured
+    from the cold run's iteration-0 residual."""
+    scheme = scheme_params(2, "ausm", "lsq", "bj")
+    t0 = time.perf_counter()
+    X, T, _, t_gen = training_set(conditions(range(n_train)), nx_train, nx_train // 3, nx_train // 3, scheme,
+                                  train_iters, -8.0, log)
+    sur = Surrogate(X, T, stages, steps, width=width, log=log)
+    n_samples = len(X)
+    del X, T
+    cond = dict(seed=-1, mach=mach, aoa_deg=aoa_deg, h=h)
+    print(f"[warmstart] building the {nx}x{nx // 3}x{nx // 3}x6 channel", file=log, flush=True)
+    mesh = Mesh.bump(nx, nx // 3, nx // 3, h, 4, (1, 1, 1))
+    blk = mesh.block(0)
+    t1 = time.perf_counter()
+    W = sur.state(blk, cond)
+    t_infer = time.perf_counter() - t1
+    print(f"[warmstart] surrogate field of {blk['nall']} cells in {t_infer:.1f}s", file=log, flush=True)
+    cold, _, t_cold = solve_batch([mesh], [cond], scheme, iters, -20.0, chunk=500, history_every=10, log=log)
+    warm, _, t_warm = solve_batch([mesh], [cond], scheme, iters, -20.0, warm=[W], rms_ref=[cold[0]["rms0"]],
+                                  chunk=500, history_every=10, log=log)
+    ch, wh = cold[0]["history"], warm[0]["history"]
+    cold_final = ch[-1][1]
+    reach = next((it for it, d, *_ in wh if d <= cold_final), None)
+    out = dict(cells=int(blk["nreal"]), mach=mach, aoa_deg=aoa_deg, h=h, iterations=iters, n_train=n_train,
+               train_mesh=[nx_train, nx_train // 3, nx_train // 3], train_iterations=train_iters,
+               samples=int(n_samples), networks=stages, width=width, train_steps=steps,
+               generate_seconds=t_gen, train_seconds=sur.train_seconds, train_step_ms=sur.step_ms,
+               train_tflops=sur.tflops, inference_seconds=t_infer, cold_seconds=t_cold, warm_seconds=t_warm,
+               cold_final_drop=cold_final, warm_initial_drop=wh[0][1], warm_final_drop=wh[-1][1],
+               warm_iterations_to_cold_final=reach,
+               iterations_ratio_at_cold_final=(ch[-1][0] + 1) / (reach + 1) if reach is not None else None,
+               cl_cold=ch[-1][2], cl_warm=wh[-1][2], cd_cold=ch[-1][3], cd_warm=wh[-1][3],
+               cold_history=ch, warm_history=wh, wall_seconds=time.perf_counter() - t0)
+    print(f"[warmstart] {blk['nreal']} cells, {iters} iterations: cold drop {cold_final:.2f}, warm starts at "
+          f"{wh[0][1]:.2f} and ends at {wh[-1][1]:.2f}; warm reaches the cold run's final drop after {reach} "
+          f"iterations", file=log, flush=True)
+    sur.close()
+    return out
 
 
 def main(argv=None):
@@ -239,10 +374,25 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--deeper", type=float, default=3.0,
                     help="decades beyond the criterion for the converged-coefficient comparison (0: skip)")
+    ap.add_argument("--train-criterion", type=float, default=None,
+                    help="stop criterion of the training fields (default: --criterion)")
+    ap.add_argument("--width", type=int, default=256)
+    ap.add_argument("--boost", type=int, default=0,
+                    help="further networks, each fitted to what the ones before leave")
+    ap.add_argument("--large", action="store_true",
+                    help="the 37.7M-cell AUSM/LSQ/BJ channel (384x128x128, M=0.7, 3 deg, h=0.1) instead")
+    ap.add_argument("--large-iters", type=int, default=3000)
+    ap.add_argument("--train-iters", type=int, default=8000)
     ap.add_argument("--json")
     a = ap.parse_args(argv)
+    if a.large:
+        s = run_large(a.train, a.nx, a.train_iters, a.steps, 1 + a.boost, a.width, iters=a.large_iters)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(s, f, indent=1)
+        return s
     s = run(a.train, a.heldout, a.nx, a.nx // 3, a.nx // 3, a.order, a.max_iter, a.criterion, a.steps, a.batch, a.lr,
-            deeper=a.deeper)
+            width=a.width, deeper=a.deeper, train_criterion=a.train_criterion, boost=a.boost)
     print(f"[warmstart] held-out converged in both: {s['converged_both']}/{a.heldout}; cold/warm iterations "
           f"{s['iterations_ratio']:.2f}x, time {s['time_ratio']:.2f}x; Cl/Cd rel diff at the criterion "
           f"{s['max_cl_rel_diff']:.2e}/{s['max_cd_rel_diff']:.2e}, {a.deeper:g} decades deeper "

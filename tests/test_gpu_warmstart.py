@@ -1,0 +1,34 @@
+"""The learned warm start end to end, at a size that runs in seconds.
+
+Training fields from the batched solver on a 12x4x4x6 channel, two networks
+trained on the GPU (the second fitted to what the first leaves), then held-out
+conditions run from freestream and from the surrogate's field with the same
+stop test, the drop of both measured from the cold run's iteration-0
+residual. The warm runs must stop sooner, and, taken four decades further,
+reach the same lift and drag as the cold runs (1e-7): the answer is the
+method's, the network only shortens the way there.
+"""
+import numpy as np
+import pytest
+
+from euler3d_amd import warmstart as ws
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warm_start_reaches_the_criterion_sooner_with_the_same_coefficients():
+    print("warm start: 24 training fields on 12x4x4x6, 4 held-out", flush=True)
+    s = ws.run(n_train=24, n_heldout=4, nx=12, ny=4, nz=4, order=1, max_iter=20000, criterion=-6.0, steps=3000,
+               width=128, deeper=4.0, train_criterion=-6.0, boost=1)
+    assert s["converged_both"] == 4
+    rows = s["heldout"]
+    for r in rows:
+        info = {k: r[k] for k in ("seed", "cold_iterations", "warm_iterations", "warm_initial_drop",
+                                  "cl_deep_rel_diff", "cd_deep_rel_diff")}
+        print(info, flush=True)
+        assert r["warm_initial_drop"] < 0.0, info      # the surrogate starts below the cold residual
+        assert r["warm_iterations"] < r["cold_iterations"], info
+        assert r["deep_converged"], info
+        assert r["cl_deep_rel_diff"] <= 1e-7 and r["cd_deep_rel_diff"] <= 1e-7, info
+    assert s["iterations_ratio"] > 1.2, s["iterations_ratio"]
+    assert np.isfinite(s["train_mean_err"]) and s["train_mean_err"] < 0.01
