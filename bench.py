@@ -23,8 +23,8 @@ cpu_baseline_1core: block 0 (589,824 cells) alone on one core, 4 iterations.
 Other workloads: --scheme ausm-lsq-bj --nx 384 --ny 128 --nz 128 --parts 1 1 1
 is the 37,748,736-cell single block with AUSM, least squares and
 Barth-Jespersen; --brick -S numbers the tets in a pseudo-random order (seed S)
-and --order rcm renumbers cells and faces in device memory by reverse
-Cuthill-McKee (the same results bit for bit).
+and --order rcm / morton renumbers cells and faces in device memory by reverse
+Cuthill-McKee or along a Morton curve (the same results bit for bit).
 """
 from __future__ import annotations
 
@@ -141,7 +141,7 @@ def main():
     ap.add_argument("--brick", type=int, default=4,
                     help="cell numbering of the generated mesh: bricks of brick^3 hexes (0 lexicographic), "
                          "or < 0 for tets in a pseudo-random order seeded by -brick")
-    ap.add_argument("--order", choices=["reference", "rcm"], default="reference",
+    ap.add_argument("--order", choices=["reference", "rcm", "morton"], default="reference",
                     help="device memory order of cells and faces (results are the same bit for bit)")
     ap.add_argument("--graph", action="store_true",
                     help="advance through the batch API with one instance: stop test on the device and a "
@@ -172,7 +172,7 @@ def main():
                            criterion=-1e9)
     ctx = E.Context(0)
     ctx.set_params(params)
-    ctx.set_ordering(E.ORDER_RCM if a.order == "rcm" else E.ORDER_REFERENCE)
+    ctx.set_ordering({"reference": E.ORDER_REFERENCE, "rcm": E.ORDER_RCM, "morton": E.ORDER_MORTON}[a.order])
     print(f"[bench] mesh built in {t_mesh:.1f}s; uploading", file=sys.stderr, flush=True)
     if a.graph:
         ctx.upload_batch([mesh], [dict(mach=sc["mach"], aoa_deg=sc["aoa"], criterion=-1e9, max_iter=10 ** 9)])

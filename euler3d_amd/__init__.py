@@ -19,7 +19,7 @@ FLUX_ROE, FLUX_AUSM = 0, 1
 GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
 LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1
 ARR_W, ARR_W0, ARR_CONV, ARR_DISS, ARR_GRAD, ARR_LIM, ARR_DT, ARR_SPECRAD = range(8)
-ORDER_REFERENCE, ORDER_RCM = 0, 1
+ORDER_REFERENCE, ORDER_RCM, ORDER_MORTON = 0, 1, 2
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
@@ -97,6 +97,7 @@ def lib():
             "e3d_mesh_nblocks": (C.c_int, [C.c_void_p]),
             "e3d_mesh_block": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BlockView)]),
             "e3d_mesh_free": (None, [C.c_void_p]),
+            "e3d_mesh_layout": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),
             "e3d_read_input": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int]),
             "e3d_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
             "e3d_destroy": (None, [C.c_void_p]),
@@ -223,6 +224,15 @@ class Mesh:
         out["node_xyz"] = _arr(v.node_xyz, 3 * v.nnode, np.float64)
         return out
 
+    def layout(self, b, mode):
+        """(cell_pos, face_pos): device positions of block b's real cells and faces under
+        a memory order (ORDER_REFERENCE, ORDER_RCM, ORDER_MORTON)."""
+        v = self.block(b)
+        cp = np.empty(v["nreal"], np.int32)
+        fp = np.empty(v["nface"], np.int32)
+        _check(lib().e3d_mesh_layout(self._h, b, mode, cp.ctypes.data_as(_ip), fp.ctypes.data_as(_ip)))
+        return cp, fp
+
     def close(self):
         if self._h and self._h.value:
             lib().e3d_mesh_free(self._h)
@@ -317,7 +327,7 @@ class Context:
         self._c(lib().e3d_set_tile_order(self._h, 1 if xcd_contiguous else 0))
 
     def set_ordering(self, mode):
-        """Memory order of the next upload: ORDER_REFERENCE or ORDER_RCM."""
+        """Memory order of the next upload: ORDER_REFERENCE, ORDER_RCM or ORDER_MORTON."""
         self._c(lib().e3d_set_ordering(self._h, mode))
 
     def init_freestream(self):

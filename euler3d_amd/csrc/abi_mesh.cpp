@@ -1,4 +1,5 @@
 // C ABI: errors, host mesh building and the input-file parser. No HIP here.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -8,6 +9,7 @@
 #include "../../include/euler3d.h"
 #include "abi_internal.hpp"
 #include "mesh.hpp"
+#include "ordering.hpp"
 #include "mesh_internal.hpp"
 
 namespace e3d {
@@ -143,6 +145,19 @@ int e3d_mesh_write(const e3d_mesh* m, const char* dir, const char* stem) {
 }
 
 int e3d_mesh_nblocks(const e3d_mesh* m) { return m ? (int)m->built.size() : E3D_EINVAL; }
+
+int e3d_mesh_layout(const e3d_mesh* m, int b, int mode, int* cell_pos, int* face_pos) {
+    if (!m || b < 0 || b >= (int)m->built.size() || !cell_pos || !face_pos)
+        return set_error(nullptr, E3D_EINVAL, "e3d_mesh_layout: bad argument");
+    if (mode != E3D_ORDER_REFERENCE && mode != E3D_ORDER_RCM && mode != E3D_ORDER_MORTON)
+        return set_error(nullptr, E3D_EINVAL, "e3d_mesh_layout: unknown mode " + std::to_string(mode));
+    const HostBlock& B = m->built[b];
+    std::vector<int> cp, fp;
+    block_layout(B, mode, cp, fp);
+    std::copy(cp.begin(), cp.end(), cell_pos);
+    std::copy(fp.begin(), fp.end(), face_pos);
+    return E3D_OK;
+}
 
 int e3d_mesh_block(const e3d_mesh* m, int b, e3d_block_view* v) {
     if (!m || !v || b < 0 || b >= (int)m->built.size())

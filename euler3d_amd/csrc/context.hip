@@ -1152,10 +1152,12 @@ int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
     return n;
 }
 
-static_assert(E3D_ORDER_REFERENCE == ORDER_REFERENCE && E3D_ORDER_RCM == ORDER_RCM, "ordering modes");
+static_assert(E3D_ORDER_REFERENCE == ORDER_REFERENCE && E3D_ORDER_RCM == ORDER_RCM &&
+                  E3D_ORDER_MORTON == ORDER_MORTON,
+              "ordering modes");
 int e3d_set_ordering(e3d_ctx* c, int mode) {
     if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
-    if (mode != E3D_ORDER_REFERENCE && mode != E3D_ORDER_RCM)
+    if (mode != E3D_ORDER_REFERENCE && mode != E3D_ORDER_RCM && mode != E3D_ORDER_MORTON)
         return set_error(c, E3D_EINVAL, "e3d_set_ordering: unknown mode " + std::to_string(mode));
     c->ordering = mode;
     return E3D_OK;

@@ -4,13 +4,16 @@
 // appearance over those cells (BlockBuilder/src/BlockBuilder.cpp:147-458). That
 // order fixes every per-cell summation order, so it is never changed: each
 // cell's face slots stay in ascending reference face id. What may change is
-// where a cell and a face sit in HBM. Reverse Cuthill-McKee places cells that
-// share faces close together, so the neighbour gathers of a wavefront touch few
+// where a cell and a face sit in HBM. Reverse Cuthill-McKee (over the face
+// graph) or a Morton curve (over the cell centroids) places cells that share
+// faces close together, so the neighbour gathers of a wavefront touch few
 // cache lines whatever order the mesh file came in.
 #include "ordering.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <numeric>
+#include <utility>
 
 namespace e3d {
 
@@ -94,6 +97,42 @@ std::vector<int> rcm_order(const HostBlock& B) {
     return order;
 }
 
+namespace {
+// Bits of v (< 2^21) spread to every third bit.
+uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+}  // namespace
+
+std::vector<int> morton_order(const HostBlock& B) {
+    const int n = B.nreal;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int q = 0; q < n; ++q)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], B.ccenter[3 * q + d]);
+            hi[d] = std::max(hi[d], B.ccenter[3 * q + d]);
+        }
+    // one scale for all three axes, so the curve's cells are cubes
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    const double sc = ext > 0.0 ? (double)((1 << 21) - 1) / ext : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (int q = 0; q < n; ++q) {
+        uint64_t k = 0;
+        for (int d = 0; d < 3; ++d) k |= spread3((uint64_t)((B.ccenter[3 * q + d] - lo[d]) * sc)) << d;
+        key[q] = {k, q};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> order(n);
+    for (int p = 0; p < n; ++p) order[p] = key[p].second;
+    return order;
+}
+
 void block_layout(const HostBlock& B, int mode, std::vector<int>& cell_pos, std::vector<int>& face_pos) {
     const int n = B.nreal;
     cell_pos.resize(n);
@@ -103,7 +142,7 @@ void block_layout(const HostBlock& B, int mode, std::vector<int>& cell_pos, std:
         std::iota(face_pos.begin(), face_pos.end(), 0);
         return;
     }
-    const std::vector<int> order = rcm_order(B);
+    const std::vector<int> order = mode == ORDER_MORTON ? morton_order(B) : rcm_order(B);
     for (int p = 0; p < n; ++p) cell_pos[order[p]] = p;
     // faces by first appearance over the cells in their new order
     std::fill(face_pos.begin(), face_pos.end(), -1);

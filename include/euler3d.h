@@ -262,8 +262,9 @@ int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
  * E3D_ORDER_REFERENCE (default) keeps the reference's numbering
  * (BlockBuilder/src/BlockBuilder.cpp:147-458, cells as the mesh file lists them,
  * faces by first appearance). E3D_ORDER_RCM places each block's real cells in
- * reverse Cuthill-McKee order and its faces by first appearance over them, so
- * that neighbour gathers stay local whatever order the mesh file came in. Only
+ * reverse Cuthill-McKee order, E3D_ORDER_MORTON along a Morton (Z-order) curve
+ * of their centroids, and the faces by first appearance over them, so that
+ * neighbour gathers stay local whatever order the mesh file came in. Only
  * addresses change: every cell keeps its faces in the reference's order, so
  * every per-cell value is bit for bit the same; the residual norms, summed
  * over cells in memory order, agree to rounding. Arrays are still read and
@@ -272,7 +273,13 @@ int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
  * DataStructure/head/Block.h:17-60). */
 #define E3D_ORDER_REFERENCE 0
 #define E3D_ORDER_RCM 1
+#define E3D_ORDER_MORTON 2
 int e3d_set_ordering(e3d_ctx* ctx, int mode);
+/* The device positions that mode gives one block of a host mesh:
+ * cell_pos[q] for its nreal real cells, face_pos[f] for its nface faces (both
+ * permutations, 0-based within the block). Host only; for callers that keep
+ * their own arrays in device order. */
+int e3d_mesh_layout(const e3d_mesh* mesh, int block, int mode, int* cell_pos, int* face_pos);
 
 /* ------------------------------------------------------ learned warm start
  * A coordinate MLP that predicts an initial flow field, replacing the uniform

@@ -1,7 +1,7 @@
 """Memory order of cells and faces on the device (e3d_set_ordering).
 
-Reverse Cuthill-McKee renumbers each block's real cells and faces in device
-memory only; every cell keeps its faces in the reference's order, so every
+Reverse Cuthill-McKee or a Morton curve renumbers each block's real cells and
+faces in device memory only; every cell keeps its faces in the reference's order, so every
 per-cell value is the same bit for bit as in the reference's numbering. The
 residual norms sum over cells in memory order and agree to 1e-12.
 
@@ -45,7 +45,7 @@ def rel_close(a, b, rtol=1e-12):
 
 @pytest.mark.parametrize("meta", [SECOND_ORDER, dict(SECOND_ORDER, flux=1, grad=1, lim=0, mach=0.7, aoa=3.0)],
                          ids=["roe_gg_vk", "ausm_lsq_bj"])
-def test_rcm_scrambled_mesh_equals_reference_order_and_oracle(meta):
+def test_scrambled_mesh_equal_in_every_order_and_to_oracle(meta):
     mesh = E.Mesh.bump(16, 6, 6, 0.1, -7, (2, 1, 1))
     print("scrambled 16x6x6 mesh, 2 blocks, 4 iterations", flush=True)
     ref_arrays, ref_hist = run(mesh, meta, E.ORDER_REFERENCE, 4)
@@ -53,6 +53,11 @@ def test_rcm_scrambled_mesh_equals_reference_order_and_oracle(meta):
     for key, a in ref_arrays.items():
         assert (bits(a) == bits(rcm_arrays[key])).all(), key
     for a, b in zip(ref_hist, rcm_hist):
+        assert rel_close(b["rms"], a["rms"]) and rel_close([b["cl"], b["cd"]], [a["cl"], a["cd"]])
+    mort_arrays, mort_hist = run(mesh, meta, E.ORDER_MORTON, 4)
+    for key, a in ref_arrays.items():
+        assert (bits(a) == bits(mort_arrays[key])).all(), key
+    for a, b in zip(ref_hist, mort_hist):
         assert rel_close(b["rms"], a["rms"]) and rel_close([b["cl"], b["cd"]], [a["cl"], a["cd"]])
     # the oracle on the same (scrambled) numbering
     geoms = [mesh.block(b) for b in range(mesh.nblocks)]
@@ -67,14 +72,14 @@ def test_rcm_scrambled_mesh_equals_reference_order_and_oracle(meta):
         assert rel_close(a["rms"], r["rms"])
 
 
-def test_rcm_batch_equals_reference_order():
+def test_batch_equal_in_every_order():
     scheme = dict(flux=E.FLUX_ROE, order=2, gradient=E.GRAD_GREEN_GAUSS, limiter=E.LIM_VENKATAKRISHNAN, k=5.0,
                   stages=5)
     meshes = [E.Mesh.bump(8, 3, 3, 0.1, -3, (1, 1, 1)), E.Mesh.bump(12, 4, 4, 0.06, 4, (2, 1, 1))]
     insts = [dict(mach=0.5, aoa_deg=1.0, criterion=-15.0, max_iter=12),
              dict(mach=0.7, aoa_deg=-1.0, criterion=-15.0, max_iter=12)]
     out = []
-    for ordering in (E.ORDER_REFERENCE, E.ORDER_RCM):
+    for ordering in (E.ORDER_REFERENCE, E.ORDER_RCM, E.ORDER_MORTON):
         ctx = E.Context(0)
         ctx.set_params(E.make_params(0.5, 0.0, **scheme))
         ctx.set_ordering(ordering)
@@ -83,8 +88,8 @@ def test_rcm_batch_equals_reference_order():
         ctx.batch_run(12, use_graph=True)
         out.append([ctx.get(b, E.ARR_W) for b in range(3)])
         ctx.close()
-    for a, b in zip(*out):
-        assert (bits(a) == bits(b)).all()
+    for a, b, c in zip(*out):
+        assert (bits(a) == bits(b)).all() and (bits(a) == bits(c)).all()
 
 
 def test_set_ordering_rejects_unknown_mode():

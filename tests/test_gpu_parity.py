@@ -37,7 +37,7 @@ def close_rel(a, b, rtol=MON_RTOL):
     return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-300)) or np.allclose(a, b, rtol=rtol, atol=1e-280)
 
 
-ORDERINGS = {"reference_order": E.ORDER_REFERENCE, "rcm_order": E.ORDER_RCM}
+ORDERINGS = {"reference_order": E.ORDER_REFERENCE, "rcm_order": E.ORDER_RCM, "morton_order": E.ORDER_MORTON}
 
 
 @pytest.mark.parametrize("ordering", list(ORDERINGS))
