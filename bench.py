@@ -18,7 +18,9 @@ time from HIP events recorded on the context's stream during the timed region;
 cpu_baseline: the oracle (plain-C restatement of the reference's solver loop)
 on the host cores, on a bounded sample of the same workload: all 8 blocks, one
 thread per block (the reference's MPI decomposition), 4 RK iterations;
-cpu_baseline_1core: block 0 (589,824 cells) alone on one core, 4 iterations.
+cpu_baseline_1core: block 0 (589,824 cells) alone on one core, 4 iterations;
+cpu_baseline_reference: the reference's own serial binary (oracle/_ref) on a
+9,216-cell channel of the same scheme, one core, its solve timer.
 
 Other workloads: --scheme ausm-lsq-bj --nx 384 --ny 128 --nz 128 --parts 1 1 1
 is the 37,748,736-cell single block with AUSM, least squares and
@@ -128,6 +130,59 @@ def cpu_baselines(a, sc, blocks):
     return cpu, cpu1
 
 
+FLUX_NAMES = {0: "roe", 1: "ausm"}
+GRAD_NAMES = {0: "greengauss", 1: "leastsquares"}
+LIM_NAMES = {0: "barthjespersen", 1: "venkatakrishnan"}
+
+
+def reference_baseline(sc, nx=24, iters=40):
+    """The reference's own serial binary (compiled from its sources into
+    oracle/_ref by oracle/Makefile; absent: None) on a small channel of the same
+    scheme, one core: its mesh setup is quadratic in the face count (about 17 s
+    at 9,216 cells), so only its own solve timer (main.cpp:89-95) is used."""
+    import re
+    import subprocess
+    import tempfile
+    import euler3d_amd as E
+    exe = os.path.join(ROOT, "oracle", "_ref", "euler3D")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        mesh = E.Mesh.bump(nx, nx // 3, nx // 3, 0.1, 4, (1, 1, 1))
+        n = mesh.block(0)["nreal"]
+        top = mesh.write(d, "bump")
+        lines = ["MESH", "topologyfilename", top, "INPUT", "cfl gamma angleattackdeg", f"1.5 1.4 {sc['aoa']!r}",
+                 "rkstage mach cmac", f"5 {sc['mach']!r} 1.0", "nbitermax convcriterion", f"{iters} -1e9",
+                 "residual smoothing (0-no 1-yes)", "0", "fluxscheme schemeorder",
+                 f"{FLUX_NAMES[sc['flux']]} {sc.get('order', 2)}", "gradient limiter",
+                 f"{GRAD_NAMES[sc['gradient']]} {LIM_NAMES[sc['limiter']]}", "omega k",
+                 f"{sc['omega']!r} {sc['k']!r}", "EXECUTABLE (0-no 1-yes)", "build execute", "1 1",
+                 "OUTPUT (0-no 1-yes)", "clalpha cdalpha cmalpha", "0 0 0", "coefficientsconv  residualconv", "1 1",
+                 "cpxc machcontour cpcontour machisosurface", "0 0 0 0", "slicecp", "0",
+                 "axis xcoord ycoord zcoord", "x 1.00 1.00 1.00"]
+        with open(os.path.join(d, "input"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        os.makedirs(os.path.join(d, "bin"))
+        print(f"[bench] reference binary on the {nx}x{nx // 3}x{nx // 3}x6 channel ({n} cells), {iters} iterations",
+              file=sys.stderr, flush=True)
+        try:
+            r = subprocess.run([exe, "input"], cwd=d, capture_output=True, text=True, timeout=600)
+        except (OSError, subprocess.TimeoutExpired) as e:
+            print(f"[bench] reference binary did not run: {e}", file=sys.stderr, flush=True)
+            return None
+        m = re.search(r"Time elapsed solving: ([0-9.eE+-]+)s", r.stdout)
+        if r.returncode != 0 or not m:
+            print(f"[bench] reference binary failed (status {r.returncode})", file=sys.stderr, flush=True)
+            return None
+        t = float(m.group(1))
+    out = {"value": n * 5 * iters / t / 1e6, "unit": "Mcell-RK-stage/s", "cores": 1, "kind": "reference",
+           "sample": f"the reference's serial binary (oracle/_ref, built from its sources) on the same scheme on the "
+                     f"{nx}x{nx // 3}x{nx // 3}x6 channel ({n} cells, one block), {iters} RK iterations in {t:.2f}s "
+                     f"by its own solve timer; its O(F^2) mesh setup excluded"}
+    print(f"[bench] reference binary: {out['value']:.3f} Mcell-RK-stage/s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,9 +290,10 @@ def main():
         hip.hipFree(dst)
         print(f"[bench] calibration copy of 1 GiB: {'done' if ok else 'FAILED'}", file=sys.stderr, flush=True)
 
-    cpu, cpu1 = None, None
+    cpu, cpu1, cpu_ref = None, None, None
     if not a.no_cpu_baseline:
         cpu, cpu1 = cpu_baselines(a, sc, blocks)
+        cpu_ref = reference_baseline(sc)
 
     out = {
         "metric": "Mcell-RK-stage/s (FP64, %HBM roof) + time to 8-order residual drop, warm vs cold",
@@ -256,6 +312,7 @@ def main():
         "kernels": kt,
         "cpu_baseline": cpu,
         "cpu_baseline_1core": cpu1,
+        "cpu_baseline_reference": cpu_ref,
     }
     print(json.dumps(out))
 
