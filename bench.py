@@ -175,11 +175,49 @@ def reference_baseline(sc, nx=24, iters=40):
             print(f"[bench] reference binary failed (status {r.returncode})", file=sys.stderr, flush=True)
             return None
         t = float(m.group(1))
+    # the oracle on the identical sample, for the oracle-to-reference speed ratio
+    import oracle as O
+    S = O.Solver([mesh.block(0)], O.params(sc["mach"], sc["aoa"], flux=sc["flux"], order=sc.get("order", 2),
+                                          gradient=sc["gradient"], limiter=sc["limiter"], k=sc["k"],
+                                          omega=sc["omega"], stages=5, max_iter=10 ** 9, criterion=-1e9))
+    S.init_freestream()
+    c0 = time.perf_counter()
+    S.iterate(iters)
+    t_or = time.perf_counter() - c0
     out = {"value": n * 5 * iters / t / 1e6, "unit": "Mcell-RK-stage/s", "cores": 1, "kind": "reference",
+           "oracle_same_sample": n * 5 * iters / t_or / 1e6,
            "sample": f"the reference's serial binary (oracle/_ref, built from its sources) on the same scheme on the "
                      f"{nx}x{nx // 3}x{nx // 3}x6 channel ({n} cells, one block), {iters} RK iterations in {t:.2f}s "
                      f"by its own solve timer; its O(F^2) mesh setup excluded"}
     print(f"[bench] reference binary: {out['value']:.3f} Mcell-RK-stage/s", file=sys.stderr, flush=True)
+    return out
+
+
+def stream_bandwidth(gib=4, reps=10):
+    """Achievable HBM bandwidth on this GPU for three traffic mixes, with
+    PyTorch's own kernels: copy (half reads, half writes), sum (reads) and fill
+    (writes) of a gib-GiB FP64 array, HIP events around reps launches."""
+    import torch
+    dev = torch.device("cuda:0")
+    n = gib * (1 << 30) // 8
+    a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: b.copy_(a), 2 * n * 8), ("read", lambda: a.sum(), n * 8),
+                             ("write", lambda: b.fill_(2.0), n * 8)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": ms, "GB/s": nbytes / ms / 1e6}
+        print(f"[bench] {name}: {ms:.3f} ms per {gib} GiB launch = {nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr,
+              flush=True)
     return out
 
 
@@ -206,7 +244,12 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy: the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--stream-bw", action="store_true",
+                    help="only measure the achievable HBM bandwidth of copy / read / write streams and exit")
     a = ap.parse_args()
+    if a.stream_bw:
+        print(json.dumps({"stream_bandwidth": stream_bandwidth(), "peak_GB/s": HBM_PEAK_GBS}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (blocks would shard "
               "across GPUs by contiguous ranges, but that path is not built)", file=sys.stderr)
