@@ -94,3 +94,11 @@ def test_morton_follows_the_z_curve_of_the_centroids():
     bw_scr, mean_scr = spread(np.arange(scr.block(0)["nreal"]), pairs)
     cpm, _ = scr.layout(0, E.ORDER_MORTON)
     assert spread(cpm, pairs)[1] < mean_scr / 20
+
+
+def test_layout_rejects_bad_arguments():
+    m = E.Mesh.bump(4, 2, 2, 0.1, 4, (1, 1, 1))
+    with pytest.raises(E.E3DError):
+        m.layout(0, 7)
+    with pytest.raises(E.E3DError):
+        m.layout(3, E.ORDER_RCM)
