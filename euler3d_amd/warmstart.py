@@ -58,12 +58,22 @@ def freestream(mach, aoa_deg):
     return np.array([1.0, mach * math.sqrt(GAMMA) * math.cos(a), mach * math.sqrt(GAMMA) * math.sin(a), 0.0, 1.0])
 
 
-def features(block, cond):
+def features(block, cond, chunk=1 << 20):
     """Network inputs of every cell of a block (real cells, then ghosts), about [-1, 1]: flow
     parameters, centroid, distance above the wall (linear and logarithmic), and sines and
     cosines of the streamwise coordinate (4 octaves), the wall-normal one (3) and the spanwise
-    one (1), so that the network can place the sharp features near the bump."""
+    one (1), so that the network can place the sharp features near the bump. Large blocks
+    are done in chunks on a thread pool (numpy releases the GIL inside its loops)."""
     c = block["ccenter"].reshape(-1, 3)
+    if len(c) <= chunk:
+        return _features(c, cond)
+    from concurrent.futures import ThreadPoolExecutor
+    parts = [c[i:i + chunk] for i in range(0, len(c), chunk)]
+    with ThreadPoolExecutor(16) as pool:
+        return np.concatenate(list(pool.map(lambda p: _features(p, cond), parts)))
+
+
+def _features(c, cond):
     x, y, z = c[:, 0], c[:, 1], c[:, 2]
     d = y - wall_y(x, cond["h"])
     f = np.empty((len(c), N_FEATURES), np.float32)
