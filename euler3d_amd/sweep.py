@@ -49,13 +49,15 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
     from . import autotune as at
     cases = instances(n, seed0, scale)
     tile_votes = {0: 0.0, 1: 0.0}
+    mem_votes = {}
     t0 = time.perf_counter()
     for c in cases:
         sc = at.size_class(c["nx"])
-        brick, order = at.tune(sc, sc // 3, sc // 3, SCHEME) if autotune else (4, 1)
+        brick, order, mem = at.tune(sc, sc // 3, sc // 3, SCHEME, verbose=bool(log)) if autotune else (4, 1, 0)
         c["brick"] = brick
         c["cells"] = 6 * c["nx"] * c["ny"] * c["nz"]
         tile_votes[order] += c["cells"]
+        mem_votes[mem] = mem_votes.get(mem, 0.0) + c["cells"]
     t_tune = time.perf_counter() - t0
     # the mesh builder runs outside the interpreter lock: build the instances in parallel
     from concurrent.futures import ThreadPoolExecutor
@@ -63,8 +65,10 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
         meshes = list(pool.map(lambda c: Mesh.bump(c["nx"], c["ny"], c["nz"], c["h"], c["brick"], (1, 1, 1)), cases))
     t_mesh = time.perf_counter() - t0 - t_tune
     tile_order = max(tile_votes, key=tile_votes.get)
+    memory_order = max(mem_votes, key=mem_votes.get)  # one device memory order for the whole batch, by cells
     ctx = Context(0)
     ctx.set_params(make_params(0.5, 0.0, max_iter=max_iter, criterion=criterion, **SCHEME))
+    ctx.set_ordering(memory_order)
     ctx.upload_batch(meshes, [dict(mach=c["mach"], aoa_deg=c["aoa_deg"], cmac=1.0, criterion=criterion,
                                    max_iter=max_iter) for c in cases])
     ctx.set_tile_order(tile_order)
@@ -72,7 +76,7 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
     ctx.op("sync")
     if log:
         print(f"[sweep] {n} instances, {sum(c['cells'] for c in cases)} cells, tuned in {t_tune:.1f}s, meshes built "
-              f"in {t_mesh:.1f}s, tile order {tile_order}", file=log, flush=True)
+              f"in {t_mesh:.1f}s, tile order {tile_order}, memory order {memory_order}", file=log, flush=True)
     t1 = time.perf_counter()
     done, status = ctx.batch_run(max_iter, use_graph=use_graph)
     wall = time.perf_counter() - t1

@@ -135,3 +135,18 @@ def test_batch_api_errors():
     with pytest.raises(E.E3DError):
         ctx.batch_history(3, 10)     # no such instance
     ctx.close()
+
+
+def test_autotuner_picks_a_measured_configuration(tmp_path):
+    from euler3d_amd import autotune as at
+    cache = str(tmp_path / "tuning.json")
+    print("autotune 32x10x10", flush=True)
+    brick, tile, mem = at.tune(32, 10, 10, dict(SCHEME), cache=cache)
+    assert brick in at.BRICKS and tile in at.TILE_ORDERS and mem in at.MEMORY_ORDERS
+    import json
+    table = json.load(open(cache))
+    (entry,) = table.values()
+    assert len(entry["trials"]) == len(at.BRICKS) * len(at.TILE_ORDERS) * len(at.MEMORY_ORDERS)
+    best = min(entry["trials"], key=lambda t: t["ms_per_iteration"])
+    assert (best["brick"], best["tile_order"], best["memory_order"]) == (brick, tile, mem)
+    assert at.tune(32, 10, 10, dict(SCHEME), cache=cache) == (brick, tile, mem)  # cached
