@@ -113,6 +113,7 @@ class CompleteMesh {
     std::string topology_file_name_;
     e3d_ctx* ctx_ = nullptr;
     e3d_mesh* mesh_ = nullptr;
+    int memory_order_ = E3D_ORDER_REFERENCE;  // device memory order of cells and faces (e3d_set_ordering)
     CompleteMesh(int n_blocks, int n_blocks_in_process, int* my_blocks, std::string topology_file_name, int device = 0);
     ~CompleteMesh();
     // Reads, builds and uploads every block, then freestream everywhere

@@ -2,9 +2,11 @@
 // (input file → Interface → blocks → CompleteMesh → Solver::solve → elapsed
 // time) over the MI355X device path. Usage:
 //   euler3d <input_file> [--granular] [--quiet] [--device N] [--out DIR]
-//           [--restart DIR] [--no-solution]
+//           [--restart DIR] [--no-solution] [--order reference|rcm|morton]
 // --granular runs the operator-by-operator loop of Solver/src/Solver.cpp:76-98
-// instead of the stage-fused kernels (same results). --restart resumes from the
+// instead of the stage-fused kernels (same results). --order renumbers cells
+// and faces in device memory (reverse Cuthill-McKee or a Morton curve) for
+// meshes whose file order has little locality (same results). --restart resumes from the
 // RestartFile<block>.dat files a previous run wrote into DIR when it stopped
 // (state, iteration counter and reference residual); --no-solution skips the
 // FlowData / SurfaceFlowData / RestartFile files at the stop. Exit status 42 on
@@ -19,7 +21,7 @@
 int main(int argc, char* argv[]) {
     const char* input = nullptr;
     bool granular = false, quiet = false, solution = true;
-    int device = 0;
+    int device = 0, memory_order = E3D_ORDER_REFERENCE;
     std::string out_dir = "./bin/", restart_dir;
     for (int i = 1; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--granular")) granular = true;
@@ -28,6 +30,16 @@ int main(int argc, char* argv[]) {
         else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out_dir = std::string(argv[++i]) + "/";
         else if (!std::strcmp(argv[i], "--restart") && i + 1 < argc) restart_dir = argv[++i];
         else if (!std::strcmp(argv[i], "--no-solution")) solution = false;
+        else if (!std::strcmp(argv[i], "--order") && i + 1 < argc) {
+            const char* o = argv[++i];
+            if (!std::strcmp(o, "reference")) memory_order = E3D_ORDER_REFERENCE;
+            else if (!std::strcmp(o, "rcm")) memory_order = E3D_ORDER_RCM;
+            else if (!std::strcmp(o, "morton")) memory_order = E3D_ORDER_MORTON;
+            else {
+                std::cerr << "unknown memory order " << o << " (reference, rcm or morton)" << std::endl;
+                return 2;
+            }
+        }
         else if (!input) input = argv[i];
         else {
             std::cerr << "unexpected argument " << argv[i] << std::endl;
@@ -55,6 +67,7 @@ int main(int argc, char* argv[]) {
         CompleteMesh complete_mesh(n_blocks, n_blocks_in_process, my_blocks, interface.topology_file_name_interface_,
                                    device);
         delete[] my_blocks;
+        complete_mesh.memory_order_ = memory_order;
         complete_mesh.InitializeMyBlocks(&interface, &communicator);
         if (!restart_dir.empty()) readRestartFiles(restart_dir, &complete_mesh);
         communicator.initializeBuffers();

@@ -101,6 +101,7 @@ void CompleteMesh::InitializeMyBlocks(Interface* interface, BlockCommunicator*) 
     e3d_check(e3d_mesh_read(topology_file_name_.c_str(), &mesh_));
     if (e3d_mesh_nblocks(mesh_) != n_blocks_) throw Euler3dError(E3D_EIO, "topology block count mismatch");
     e3d_check(e3d_set_params(ctx_, &interface->params_), ctx_);
+    e3d_check(e3d_set_ordering(ctx_, memory_order_), ctx_);
     e3d_check(e3d_upload_mesh(ctx_, mesh_), ctx_);
     for (int b = 0; b < n_blocks_; ++b) {
         e3d_block_view v;

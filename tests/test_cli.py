@@ -44,6 +44,11 @@ def test_cli_missing_topology_is_io_error(tmp_path):
     assert r.returncode == 42, r.stdout + r.stderr
 
 
+def test_cli_unknown_memory_order_is_rejected(tmp_path):
+    r = run_cli(str(tmp_path / "input"), "--order", "hilbert")
+    assert r.returncode == 2 and "unknown memory order" in r.stderr
+
+
 def test_cli_unknown_flux_scheme_is_rejected(tmp_path):
     g = Golden("bump1_roe1")
     inp = tmp_path / "input"
@@ -70,14 +75,16 @@ def expected_rows(g):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["bump8_roe2_gg_vk", "naca_hex_roe2_gg_vk", "bump1_ausm2_lsq_bj", "bump8_ausm1_irs"])
-@pytest.mark.parametrize("mode", ["fused", "granular"])
+@pytest.mark.parametrize("mode", ["fused", "granular", "fused_morton", "granular_rcm"])
 def test_cli_convergence_history(tmp_path, case, mode):
     g = Golden(case)
     top = write_case_mesh(g, str(tmp_path))
     inp = tmp_path / "input"
     write_input(inp, top, g.meta)
     out = tmp_path / "bin"
-    args = [str(inp), "--quiet", "--out", str(out)] + (["--granular"] if mode == "granular" else [])
+    args = [str(inp), "--quiet", "--out", str(out)] + (["--granular"] if mode.startswith("granular") else [])
+    if "_" in mode:  # another device memory order: the same files
+        args += ["--order", mode.split("_")[1]]
     print(f"cli {case} {mode}", flush=True)
     r = run_cli(*args, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
