@@ -21,8 +21,7 @@
 #include <string>
 #include <cmath>
 #include <stdexcept>
-#include <unordered_map>
-
+#include "face_index.hpp"
 #include "mesh.hpp"
 
 namespace e3d {
@@ -35,23 +34,7 @@ const int kTet[4][4] = {{0, 2, 1, -1}, {0, 1, 3, -1}, {1, 2, 3, -1}, {2, 0, 3, -
 const int kHex[6][4] = {{1, 0, 3, 2}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {0, 1, 5, 4}, {4, 5, 6, 7}};
 const int kPyr[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {3, 0, 4, -1}};
 
-struct FaceKey {
-    int v[4];
-    bool operator==(const FaceKey& o) const {
-        return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3];
-    }
-};
-struct FaceKeyHash {
-    size_t operator()(const FaceKey& k) const {
-        uint64_t h = 1469598103934665603ull;
-        for (int i = 0; i < 4; ++i) {
-            h ^= (uint64_t)(uint32_t)k.v[i];
-            h *= 1099511628211ull;
-            h ^= h >> 29;
-        }
-        return (size_t)h;
-    }
-};
+using FaceKey = FaceIndex<4>::Key;
 FaceKey make_key(const int* n, int nn) {
     FaceKey k{{INT_MAX, INT_MAX, INT_MAX, INT_MAX}};
     for (int i = 0; i < nn; ++i) k.v[i] = n[i];
@@ -97,32 +80,44 @@ HostBlock build_block(const RawBlock& raw) {
         B.bnds.push_back(std::move(bd));
     }
 
+    for (int v : B.cell_nodes)
+        if (v < 0 || v >= B.nnode) throw std::runtime_error("mesh: element node index out of range");
+
     // Faces by first appearance over real cells.
-    std::unordered_map<FaceKey, int, FaceKeyHash> face_of;
-    face_of.reserve((size_t)nreal * 3);
-    std::vector<int> occ_cells;   // per face: cells holding it (at most 2 expected)
-    std::vector<int> occ_count;
-    for (int c = 0; c < nreal; ++c) {
+    auto face_nodes_of = [&](int c, int j, int fn[4]) {
         const int t = raw.etype[c];
         const int(*tab)[4] = t == 10 ? kTet : t == 12 ? kHex : kPyr;
         const int* cn = &B.cell_nodes[B.cell_nodes_off[c]];
+        int nn = 0;
+        for (int q = 0; q < 4; ++q)
+            if (tab[j][q] >= 0) fn[nn++] = cn[tab[j][q]];
+        return nn;
+    };
+    FaceIndex<4> face_of(B.nnode, [&](auto&& visit) {
+        for (int c = 0; c < nreal; ++c)
+            for (int j = 0; j < nfaces[c]; ++j) {
+                int fn[4];
+                const int nn = face_nodes_of(c, j, fn);
+                visit(make_key(fn, nn));
+            }
+    });
+    std::vector<int> occ_cells;   // per face: cells holding it (at most 2 expected)
+    std::vector<int> occ_count;
+    for (int c = 0; c < nreal; ++c) {
         for (int j = 0; j < nfaces[c]; ++j) {
-            int fn[4], nn = 0;
-            for (int q = 0; q < 4; ++q)
-                if (tab[j][q] >= 0) fn[nn++] = cn[tab[j][q]];
+            int fn[4];
+            const int nn = face_nodes_of(c, j, fn);
             const FaceKey key = make_key(fn, nn);
-            auto it = face_of.find(key);
-            int f;
-            if (it == face_of.end()) {
-                f = B.nface++;
-                face_of.emplace(key, f);
+            bool is_new;
+            const int f = face_of.insert_or_get(key, B.nface, &is_new);
+            if (is_new) {
+                ++B.nface;
                 B.face_nnode.push_back(nn);
                 for (int q = 0; q < 4; ++q) B.face_nodes.push_back(q < nn ? fn[q] : -1);
                 occ_cells.push_back(c);
                 occ_cells.push_back(-1);
                 occ_count.push_back(1);
             } else {
-                f = it->second;
                 if (occ_count[f] >= 2) throw std::runtime_error("mesh: face shared by more than two cells");
                 occ_cells[2 * f + occ_count[f]++] = c;
             }
@@ -130,9 +125,8 @@ HostBlock build_block(const RawBlock& raw) {
     }
     for (int gc = nreal; gc < B.nall; ++gc) {
         const FaceKey key = make_key(&B.cell_nodes[B.cell_nodes_off[gc]], B.cell_nnode[gc]);
-        auto it = face_of.find(key);
-        if (it == face_of.end()) throw std::runtime_error("mesh: boundary element matches no cell face");
-        const int f = it->second;
+        const int f = face_of.find(key);
+        if (f < 0) throw std::runtime_error("mesh: boundary element matches no cell face");
         if (occ_count[f] >= 2) throw std::runtime_error("mesh: boundary element on an interior face");
         occ_cells[2 * f + occ_count[f]++] = gc;
     }

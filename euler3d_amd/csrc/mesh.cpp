@@ -5,6 +5,7 @@
 // (read) -- blank lines after NDIME and after the node and element sections
 // are required; the topology file read in DataStructure/src/CompleteMesh.cpp:
 // 45-76 (block file names) and :170-292 (per-block GhostCount and interfaces).
+#include "face_index.hpp"
 #include "mesh.hpp"
 
 #include <algorithm>
@@ -15,7 +16,6 @@
 #include <map>
 #include <sstream>
 #include <stdexcept>
-#include <unordered_map>
 
 namespace e3d {
 
@@ -45,22 +45,11 @@ double bump_y(double x, double h) {
     return y > 0.0 ? y : 0.0;
 }
 
-struct TriKey {
-    int a, b, c;
-    bool operator==(const TriKey& o) const { return a == o.a && b == o.b && c == o.c; }
-};
-struct TriKeyHash {
-    size_t operator()(const TriKey& k) const {
-        uint64_t h = (uint64_t)(uint32_t)k.a * 0x9E3779B97F4A7C15ull;
-        h ^= (uint64_t)(uint32_t)k.b + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
-        h ^= (uint64_t)(uint32_t)k.c * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
-        return (size_t)h;
-    }
-};
+using TriKey = FaceIndex<3>::Key;
 TriKey tri_key(int a, int b, int c) {
     int v[3] = {a, b, c};
     std::sort(v, v + 3);
-    return {v[0], v[1], v[2]};
+    return {{v[0], v[1], v[2]}};
 }
 
 }  // namespace
@@ -173,27 +162,29 @@ MultiBlockMesh make_bump_channel(const BumpParams& p) {
         for (int j = 0; j < ny; ++j) add_side(nx - 1, j, k, 1, BC_FARFIELD);
 
     // Face adjacency: sorted triangle -> up to two tets.
-    std::unordered_map<TriKey, int, TriKeyHash> owner;  // key -> first tet
+    auto tet_face = [&](size_t g, int f) {
+        const int* tn = &tet_nodes[4 * g];
+        return tri_key(tn[kTetFaces[f][0]], tn[kTetFaces[f][1]], tn[kTetFaces[f][2]]);
+    };
+    FaceIndex<3> owner(NX * NY * NZ, [&](auto&& visit) {  // key -> 4 * first tet + its local face
+        for (size_t g = 0; g < ntet; ++g)
+            for (int f = 0; f < 4; ++f) visit(tet_face(g, f));
+    });
     std::vector<int> nbr(4 * ntet, -1);
-    owner.reserve(ntet * 3);
     for (size_t g = 0; g < ntet; ++g)
         for (int f = 0; f < 4; ++f) {
-            const int* tn = &tet_nodes[4 * g];
-            TriKey key = tri_key(tn[kTetFaces[f][0]], tn[kTetFaces[f][1]], tn[kTetFaces[f][2]]);
-            auto it = owner.find(key);
-            if (it == owner.end()) {
-                owner.emplace(key, (int)(4 * g + f));
-            } else {
-                const int o = it->second;
+            bool is_new;
+            const int o = owner.insert_or_get(tet_face(g, f), (int)(4 * g + f), &is_new);
+            if (!is_new) {
                 nbr[o] = (int)g;
                 nbr[4 * g + f] = o / 4;
             }
         }
     std::vector<int> btri_tet(btris.size());
     for (size_t q = 0; q < btris.size(); ++q) {
-        auto it = owner.find(tri_key(btris[q].n[0], btris[q].n[1], btris[q].n[2]));
-        if (it == owner.end()) throw std::logic_error("bump channel: boundary triangle without a tet");
-        btri_tet[q] = it->second / 4;
+        const int o = owner.find(tri_key(btris[q].n[0], btris[q].n[1], btris[q].n[2]));
+        if (o < 0) throw std::logic_error("bump channel: boundary triangle without a tet");
+        btri_tet[q] = o / 4;
     }
 
     MultiBlockMesh m;
