@@ -1,10 +1,13 @@
 // C ABI: errors, host mesh building and the input-file parser. No HIP here.
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <memory>
 #include <string>
+#include <thread>
 
 #include "../../include/euler3d.h"
 #include "abi_internal.hpp"
@@ -30,12 +33,36 @@ using namespace e3d;
 
 namespace {
 
+// Builds every block, several at a time on host threads (blocks are
+// independent); the first failure, in block order, is rethrown.
+void build_blocks(e3d_mesh& m) {
+    const size_t nb = m.raw.blocks.size();
+    m.built.resize(nb);
+    std::vector<std::exception_ptr> err(nb);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t b; (b = next.fetch_add(1)) < nb;) {
+            try {
+                m.built[b] = build_block(m.raw.blocks[b]);
+            } catch (...) {
+                err[b] = std::current_exception();
+            }
+        }
+    };
+    const size_t nt = std::min<size_t>(nb, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 int finish_mesh(std::unique_ptr<e3d_mesh>& m, e3d_mesh** out) {
     const size_t nb = m->raw.blocks.size();
-    m->built.resize(nb);
+    build_blocks(*m);
     m->flat.resize(nb);
     for (size_t b = 0; b < nb; ++b) {
-        m->built[b] = build_block(m->raw.blocks[b]);
         const HostBlock& B = m->built[b];
         auto& F = m->flat[b];
         F.bnd_off.push_back(0);
