@@ -12,7 +12,10 @@ iteration-0 residual of the same case (``rms_ref`` of a batch instance).
 Pipeline (``python -m euler3d_amd.warmstart``):
 1. training fields: ``--train`` conditions (seeds 0..n-1) solved from
    freestream by the batched solver to the stop criterion or ``--max-iter``;
-2. training: ``--steps`` Adam steps on all cells of all training fields;
+2. training: ``--steps`` Adam steps on all cells of all training fields
+   (``--boost`` further networks fitted to what the ones before leave;
+   ``--memory`` keeps what they all leave on each training field, recalled
+   for a new condition from its nearest training conditions);
 3. evaluation: ``--heldout`` conditions (seeds 5000..) run cold and warm in
    two batches of the same composition; per instance the iterations and the
    device time to the criterion, with the surrogate's inference time added to
@@ -180,17 +183,20 @@ def training_set(conds, nx, ny, nz, scheme, max_iter, criterion, log=sys.stderr)
     the criterion or max_iter), as network inputs and targets of their real cells."""
     tm = bump_meshes(conds, nx, ny, nz)
     res, states, t_gen = solve_batch(tm, conds, scheme, max_iter, criterion, log=log)
-    X, T = [], []
+    X, T, kept = [], [], []
     for m, c, W in zip(tm, conds, states):
         blk = m.block(0)
         if not np.isfinite(W).all():
             continue  # a diverged case is no training data
         X.append(features(blk, c)[:blk["nreal"]])
         T.append(targets(W, c, blk["nreal"]))
+        kept.append(c)
+    for m in tm:
+        m.close()
     X, T = np.concatenate(X), np.concatenate(T)
     print(f"[warmstart] {len(conds)} training fields ({sum(r['converged'] for r in res)} at {criterion:g}) in "
           f"{t_gen:.1f}s; {len(X)} samples", file=log, flush=True)
-    return X, T, res, t_gen
+    return X, T, res, t_gen, kept
 
 
 class Surrogate:
@@ -241,8 +247,83 @@ class Surrogate:
         self.nets = []
 
 
+def flow_params(cond):
+    """(Mach, inlet angle, bump height) scaled to about [-1, 1], as in the network's first three inputs."""
+    return np.array([(cond["mach"] - 0.55) / 0.25, (cond["aoa_deg"] - 2.0) / 4.0, (cond["h"] - 0.08) / 0.04])
+
+
+class ErrorMemory:
+    """What the surrogate leaves on each training field (target minus prediction,
+    every real cell), recalled for a new condition as a weighted sum of the
+    leftovers of its nearest training conditions in (Mach, angle, height).
+
+    The surrogate's error varies smoothly with the flow parameters, and its
+    smooth part is what the solver removes slowly; the leftovers of nearby
+    training fields carry most of it. The weights come from one of a few
+    interpolants, picked by leave-one-out over the training fields themselves:
+    ("idw", k) inverse squared distance over the k nearest, ("linear", k) the
+    intercept of a weighted least-squares plane (tricube weights) through the
+    k nearest, which is exact when the leftovers vary linearly. Needs the
+    training and the new meshes to share their cell numbering (one mesh
+    family: the bump height moves the nodes, not the topology)."""
+
+    CANDIDATES = (("idw", 1), ("idw", 2), ("idw", 4), ("idw", 8), ("idw", 16), ("linear", 8), ("linear", 12),
+                  ("linear", 16), ("linear", 24))
+
+    def __init__(self, params, R, loo_fields=128, loo_cells=16384, log=sys.stderr):
+        self.P = np.asarray(params, np.float64)   # [n x 3]
+        self.R = R                                # [n x nreal x 5], float32, target units
+        rng = np.random.default_rng(4)
+        n, nreal = R.shape[0], R.shape[1]
+        fields = rng.choice(n, size=min(n, loo_fields), replace=False)
+        cells = np.sort(rng.choice(nreal, size=min(nreal, loo_cells), replace=False))
+        Rs = R[:, cells, :]
+        base = float(np.mean([np.abs(Rs[j]).mean() for j in fields]))
+        self.loo = {"none": base}
+        for method in self.CANDIDATES:
+            if method[1] >= n:
+                continue
+            e = []
+            for j in fields:
+                idx, c = self.weights(self.P[j], method, exclude=j)
+                e.append(np.abs(Rs[j] - np.tensordot(c, Rs[idx], axes=1)).mean())
+            self.loo[f"{method[0]}{method[1]}"] = float(np.mean(e))
+        best = min(self.loo, key=self.loo.get)
+        self.method = None if best == "none" else next(m for m in self.CANDIDATES if f"{m[0]}{m[1]}" == best)
+        print(f"[warmstart] error memory of {n} fields: leave-one-out mean error "
+              + ", ".join(f"{k} {v:.3e}" for k, v in self.loo.items()) + f"; using {best}", file=log, flush=True)
+
+    def weights(self, p, method, exclude=None):
+        """Indices of the training fields used for parameters p and their weights (summing to 1)."""
+        d = np.sqrt(((self.P - p) ** 2).sum(axis=1))
+        if exclude is not None:
+            d[exclude] = np.inf
+        kind, k = method
+        idx = np.argsort(d, kind="stable")[:k]
+        dk = d[idx]
+        if kind == "idw":
+            if dk[0] == 0.0:
+                return idx[:1], np.ones(1)
+            w = 1.0 / dk ** 2
+            return idx, w / w.sum()
+        h = dk[-1] * 1.0001 + 1e-12
+        w = (1.0 - (dk / h) ** 3) ** 3
+        A = np.hstack([np.ones((k, 1)), self.P[idx] - p])
+        M = A.T @ (w[:, None] * A) + np.diag([0.0, 1e-8, 1e-8, 1e-8])
+        c = np.linalg.solve(M, A.T * w)[0]    # first row of (A'WA)^-1 A'W: the plane's value at p
+        return idx, c
+
+    def __call__(self, cond):
+        """Correction [nreal x 5] (target units) for a condition; zeros when no interpolant helped."""
+        if self.method is None:
+            return np.zeros(self.R.shape[1:], np.float32)
+        idx, c = self.weights(flow_params(cond), self.method)
+        return np.tensordot(c.astype(np.float32), self.R[idx], axes=1)
+
+
 def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000, criterion=-8.0, steps=3000,
-        batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, train_criterion=None, boost=0, log=sys.stderr):
+        batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, train_criterion=None, boost=0, memory=False,
+        log=sys.stderr):
     scheme = scheme_params(order)
     train_c = conditions(range(n_train))
     held_c = conditions(range(5000, 5000 + n_heldout))
@@ -250,21 +331,40 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
     # 1. training fields from the batched solver; they need only be converged
     # well below the network's own error
     tcrit = criterion if train_criterion is None else train_criterion
-    X, T, res, t_gen = training_set(train_c, nx, ny, nz, scheme, max_iter, tcrit, log)
+    X, T, res, t_gen, kept = training_set(train_c, nx, ny, nz, scheme, max_iter, tcrit, log)
     # 2. training on the GPU
     sur = Surrogate(X, T, 1 + boost, steps, batch, lr, width, depth, log)
     loss, t_train, ms = sur.loss, sur.train_seconds, sur.step_ms
     flops = sur.tflops * ms * 1e9
     train_max_err, train_mean_err = sur.train_max_err, sur.train_mean_err
+    n_samples = len(X)
+    mem = None
+    if memory:  # what the networks leave on every training field, for recall by nearby conditions
+        t3 = time.perf_counter()
+        R = (T - sur(X)).reshape(len(kept), len(X) // len(kept), 5)
+        mem = ErrorMemory([flow_params(c) for c in kept], R, log=log)
+        mem_build_seconds = time.perf_counter() - t3
+    del X, T
     # 3. held-out: cold, then warm from the network with the cold iteration-0 norms as reference
     hm = bump_meshes(held_c, nx, ny, nz)
     cold, cold_states, t_cold = solve_batch(hm, held_c, scheme, max_iter, criterion, history_every=50)
     t2 = time.perf_counter()
-    warm_states = [sur.state(m.block(0), c) for m, c in zip(hm, held_c)]
+    net_states = [sur.state(m.block(0), c) for m, c in zip(hm, held_c)]
+    warm_states = net_states
+    use_mem = mem is not None and mem.method is not None
+    if use_mem:
+        warm_states = []
+        for m, c, Wn in zip(hm, held_c, net_states):
+            nr = m.block(0)["nreal"]
+            if nr != mem.R.shape[1]:
+                raise ValueError("error memory: held-out mesh numbering differs from the training meshes'")
+            W = Wn.copy()
+            W[:, :nr] += (mem(c).astype(np.float64) * TARGET_SCALE).T
+            warm_states.append(W)
     t_infer = time.perf_counter() - t2
-    # the surrogates with fewer stages, for comparison (iterations to the criterion only)
+    # the surrogates with fewer stages (and all networks without the memory), for comparison (iterations only)
     by_stages = []
-    for k in range(1, boost + 1):
+    for k in range(1, boost + 1 + (1 if use_mem else 0)):
         ws_k = [sur.state(m.block(0), c, k) for m, c in zip(hm, held_c)]
         wk, _, _ = solve_batch(hm, held_c, scheme, max_iter, criterion, warm=ws_k, rms_ref=[r["rms0"] for r in cold])
         ok = [(a, b) for a, b in zip(cold, wk) if a["converged"] and b["converged"]]
@@ -282,12 +382,14 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
         deep_warm, _, _ = solve_batch(hm, held_c, scheme, 2 * max_iter, criterion - deeper, warm=warm_states,
                                       rms_ref=[r["rms0"] for r in cold])
     rows = []
-    for c, w, cs, ws, m in zip(cold, warm, cold_states, warm_states, hm):
+    for c, w, cs, ws, wn, m in zip(cold, warm, cold_states, warm_states, net_states, hm):
         nr = m.block(0)["nreal"]
         field_err = float(np.max(np.abs(ws[:, :nr] - cs[:, :nr])))  # surrogate vs converged cold state, real cells
         field_mean_err = float(np.mean(np.abs(ws[:, :nr] - cs[:, :nr])))
         rows.append(dict(seed=c["seed"], mach=c["mach"], aoa_deg=c["aoa_deg"], h=c["h"], field_max_err=field_err,
                          field_mean_err=field_mean_err,
+                         field_max_err_networks=float(np.max(np.abs(wn[:, :nr] - cs[:, :nr]))),
+                         field_mean_err_networks=float(np.mean(np.abs(wn[:, :nr] - cs[:, :nr]))),
                          cold_iterations=c["iterations"], warm_iterations=w["iterations"],
                          cold_seconds=c["stop_seconds"], warm_seconds=w["stop_seconds"] + t_infer / len(held_c),
                          cold_converged=c["converged"], warm_converged=w["converged"],
@@ -305,7 +407,9 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
     summary = dict(n_train=n_train, n_heldout=n_heldout, mesh=[nx, ny, nz], order=order, criterion=criterion,
                    train_criterion=tcrit, width=width, boost=boost, by_stages=by_stages, train_max_err=train_max_err,
                    train_mean_err=train_mean_err,
-                   max_iter=max_iter, samples=int(len(X)), train_steps=steps, batch=batch,
+                   memory=(dict(method=mem.method, leave_one_out=mem.loo, fields=int(mem.R.shape[0]),
+                                build_seconds=mem_build_seconds) if mem is not None else None),
+                   max_iter=max_iter, samples=int(n_samples), train_steps=steps, batch=batch,
                    train_seconds=t_train, train_step_ms=ms, train_tflops=flops / ms / 1e9,
                    loss_first=float(loss[0]), loss_last=float(np.mean(loss[-50:])), generate_seconds=t_gen,
                    inference_seconds=t_infer, cold_batch_seconds=t_cold, warm_batch_seconds=t_warm,
@@ -335,7 +439,7 @@ def run_large(n_train=256, nx_train=48, train_iters=8000, steps=30000, stages=3,
     from the cold run's iteration-0 residual."""
     scheme = scheme_params(2, "ausm", "lsq", "bj")
     t0 = time.perf_counter()
-    X, T, _, t_gen = training_set(conditions(range(n_train)), nx_train, nx_train // 3, nx_train // 3, scheme,
+    X, T, _, t_gen, _ = training_set(conditions(range(n_train)), nx_train, nx_train // 3, nx_train // 3, scheme,
                                   train_iters, -8.0, log)
     sur = Surrogate(X, T, stages, steps, width=width, log=log)
     n_samples = len(X)
@@ -389,6 +493,8 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--boost", type=int, default=0,
                     help="further networks, each fitted to what the ones before leave")
+    ap.add_argument("--memory", action="store_true",
+                    help="add the recalled leftovers of the nearest training fields to the networks' field")
     ap.add_argument("--large", action="store_true",
                     help="the 37.7M-cell AUSM/LSQ/BJ channel (384x128x128, M=0.7, 3 deg, h=0.1) instead")
     ap.add_argument("--large-iters", type=int, default=3000)
@@ -402,7 +508,7 @@ def main(argv=None):
                 json.dump(s, f, indent=1)
         return s
     s = run(a.train, a.heldout, a.nx, a.nx // 3, a.nx // 3, a.order, a.max_iter, a.criterion, a.steps, a.batch, a.lr,
-            width=a.width, deeper=a.deeper, train_criterion=a.train_criterion, boost=a.boost)
+            width=a.width, deeper=a.deeper, train_criterion=a.train_criterion, boost=a.boost, memory=a.memory)
     print(f"[warmstart] held-out converged in both: {s['converged_both']}/{a.heldout}; cold/warm iterations "
           f"{s['iterations_ratio']:.2f}x, time {s['time_ratio']:.2f}x; Cl/Cd rel diff at the criterion "
           f"{s['max_cl_rel_diff']:.2e}/{s['max_cd_rel_diff']:.2e}, {a.deeper:g} decades deeper "

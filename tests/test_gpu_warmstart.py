@@ -32,3 +32,23 @@ def test_warm_start_reaches_the_criterion_sooner_with_the_same_coefficients():
         assert r["cl_deep_rel_diff"] <= 1e-7 and r["cd_deep_rel_diff"] <= 1e-7, info
     assert s["iterations_ratio"] > 1.2, s["iterations_ratio"]
     assert np.isfinite(s["train_mean_err"]) and s["train_mean_err"] < 0.01
+
+
+def test_error_memory_keeps_the_answer_the_methods():
+    print("warm start with error memory: 32 training fields on 12x4x4x6, 4 held-out", flush=True)
+    s = ws.run(n_train=32, n_heldout=4, nx=12, ny=4, nz=4, order=1, max_iter=20000, criterion=-6.0, steps=3000,
+               width=128, deeper=4.0, train_criterion=-6.0, boost=0, memory=True)
+    m = s["memory"]
+    print({k: m[k] for k in ("method", "leave_one_out", "fields")}, flush=True)
+    assert m["fields"] == 32
+    # the interpolant chosen is the one with the least leave-one-out error ("none" included)
+    best = min(m["leave_one_out"], key=m["leave_one_out"].get)
+    assert (m["method"] is None) == (best == "none")
+    assert s["converged_both"] == 4
+    for r in s["heldout"]:
+        info = {k: r[k] for k in ("seed", "cold_iterations", "warm_iterations", "field_mean_err",
+                                  "field_mean_err_networks", "cl_deep_rel_diff", "cd_deep_rel_diff")}
+        print(info, flush=True)
+        assert r["warm_iterations"] < r["cold_iterations"], info
+        assert r["deep_converged"], info
+        assert r["cl_deep_rel_diff"] <= 1e-7 and r["cd_deep_rel_diff"] <= 1e-7, info
