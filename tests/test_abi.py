@@ -80,3 +80,24 @@ def test_read_input_file(tmp_path):
     f.write_text(text.replace("leastsquares venkatakrishnan", "gg bj"))
     p, _ = E.read_input(str(f))
     assert (p.gradient, p.limiter) == (E.GRAD_GREEN_GAUSS, E.LIM_BARTH_JESPERSEN)
+
+
+def test_error_in_one_block_of_several_is_reported(tmp_path):
+    """Blocks are built on host threads; a failure in any of them reaches the caller with its message."""
+    m = E.Mesh.bump(6, 2, 2, 0.1, 4, (1, 1, 1))
+    good_top = m.write(str(tmp_path), "good")
+    good = [line for line in open(good_top).read().split("\n") if line.endswith(".su2")][0]
+    su2 = tmp_path / "open.su2"
+    su2.write_text("NDIME= 3\n\nNPOIN= 4\n0 0 0\n1 0 0\n0 1 0\n0 0 1\n\nNELEM= 1\n10 0 1 2 3\n\nNMARK= 1\n"
+                   "MARKER_TAG= WALL\nMARKER_ELEMS= 1\n5 0 1 2\n")
+    blocks = [good, good, str(su2), good]
+    top = tmp_path / "four.top"
+    top.write_text(f"NBLOCK= {len(blocks)}\n" + "\n".join(blocks) + "\n" +
+                   "".join(f"Block= {b}\nGhostCount= 0\nNconnexion= 0\n" for b in range(len(blocks))))
+    with pytest.raises(E.E3DError) as e:
+        E.Mesh.read(str(top))
+    assert "unmatched" in str(e.value)
+    top.write_text(f"NBLOCK= 2\n{good}\n{good}\n" + "".join(f"Block= {b}\nGhostCount= 0\nNconnexion= 0\n"
+                                                           for b in range(2)))
+    r = E.Mesh.read(str(top))
+    assert r.nblocks == 2 and np.array_equal(r.block(0)["S"], r.block(1)["S"])
