@@ -23,6 +23,7 @@
 #include <stdexcept>
 #include "face_index.hpp"
 #include "mesh.hpp"
+#include "parallel.hpp"
 
 namespace e3d {
 
@@ -179,108 +180,120 @@ HostBlock build_block(const RawBlock& raw) {
     // ---- metrics (MetricsInitializer::doInit order) ----
     const double* X = raw.xyz.data();
     B.ccenter.assign(3 * (size_t)B.nall, 0.0);
-    for (int c = 0; c < nreal; ++c) {  // computeCenterCells :64-88
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        const unsigned nn = (unsigned)B.cell_nnode[c];
-        for (unsigned j = 0; j < nn; ++j) {
-            const int n = B.cell_nodes[B.cell_nodes_off[c] + j];
-            sx += X[3 * n];
-            sy += X[3 * n + 1];
-            sz += X[3 * n + 2];
+    parallel_for(nreal, [&](size_t c0, size_t c1) {
+        for (int c = (int)c0; c < (int)c1; ++c) {  // computeCenterCells :64-88
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const unsigned nn = (unsigned)B.cell_nnode[c];
+            for (unsigned j = 0; j < nn; ++j) {
+                const int n = B.cell_nodes[B.cell_nodes_off[c] + j];
+                sx += X[3 * n];
+                sy += X[3 * n + 1];
+                sz += X[3 * n + 2];
+            }
+            B.ccenter[3 * c] = sx / nn;
+            B.ccenter[3 * c + 1] = sy / nn;
+            B.ccenter[3 * c + 2] = sz / nn;
         }
-        B.ccenter[3 * c] = sx / nn;
-        B.ccenter[3 * c + 1] = sy / nn;
-        B.ccenter[3 * c + 2] = sz / nn;
-    }
+    });
     for (int c = nreal; c < B.nall; ++c) {  // :90-106
         const int l = B.cell2cell[B.cell_off[c]];
         for (int d = 0; d < 3; ++d) B.ccenter[3 * c + d] = B.ccenter[3 * l + d];
     }
     B.fcenter.assign(3 * (size_t)B.nface, 0.0);
-    for (int f = 0; f < B.nface; ++f) {  // computeCenterFaces :116-142
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        const unsigned nn = (unsigned)B.face_nnode[f];
-        for (unsigned j = 0; j < nn; ++j) {
-            const int n = B.face_nodes[4 * f + j];
-            sx += X[3 * n];
-            sy += X[3 * n + 1];
-            sz += X[3 * n + 2];
+    parallel_for(B.nface, [&](size_t f0, size_t f1) {
+        for (int f = (int)f0; f < (int)f1; ++f) {  // computeCenterFaces :116-142
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const unsigned nn = (unsigned)B.face_nnode[f];
+            for (unsigned j = 0; j < nn; ++j) {
+                const int n = B.face_nodes[4 * f + j];
+                sx += X[3 * n];
+                sy += X[3 * n + 1];
+                sz += X[3 * n + 2];
+            }
+            B.fcenter[3 * f] = sx / nn;
+            B.fcenter[3 * f + 1] = sy / nn;
+            B.fcenter[3 * f + 2] = sz / nn;
         }
-        B.fcenter[3 * f] = sx / nn;
-        B.fcenter[3 * f + 1] = sy / nn;
-        B.fcenter[3 * f + 2] = sz / nn;
-    }
+    });
     B.S.assign(3 * (size_t)B.nface, 0.0);
-    for (int f = 0; f < B.nface; ++f) {  // computeNormalFaces :337-448
-        const int* fn = &B.face_nodes[4 * f];
-        double s_x, s_y, s_z;
-        if (B.face_nnode[f] == 3) {
-            const double x1 = X[3 * fn[0]], x2 = X[3 * fn[1]], x3 = X[3 * fn[2]];
-            const double y1 = X[3 * fn[0] + 1], y2 = X[3 * fn[1] + 1], y3 = X[3 * fn[2] + 1];
-            const double z1 = X[3 * fn[0] + 2], z2 = X[3 * fn[1] + 2], z3 = X[3 * fn[2] + 2];
-            const double dxyA = (x1 - x2) * (y1 + y2), dxyB = (x2 - x3) * (y2 + y3), dxyC = (x3 - x1) * (y3 + y1);
-            const double dyzA = (y1 - y2) * (z1 + z2), dyzB = (y2 - y3) * (z2 + z3), dyzC = (y3 - y1) * (z3 + z1);
-            const double dzxA = (z1 - z2) * (x1 + x2), dzxB = (z2 - z3) * (x2 + x3), dzxC = (z3 - z1) * (x3 + x1);
-            s_x = 0.5 * (dyzA + dyzB + dyzC);
-            s_y = 0.5 * (dzxA + dzxB + dzxC);
-            s_z = 0.5 * (dxyA + dxyB + dxyC);
-        } else {
-            const double x1 = X[3 * fn[0]], x2 = X[3 * fn[1]], x3 = X[3 * fn[2]], x4 = X[3 * fn[3]];
-            const double y1 = X[3 * fn[0] + 1], y2 = X[3 * fn[1] + 1], y3 = X[3 * fn[2] + 1], y4 = X[3 * fn[3] + 1];
-            const double z1 = X[3 * fn[0] + 2], z2 = X[3 * fn[1] + 2], z3 = X[3 * fn[2] + 2], z4 = X[3 * fn[3] + 2];
-            const double dxA = x4 - x2, dxB = x3 - x1, dyA = y4 - y2, dyB = y3 - y1, dzA = z4 - z2, dzB = z3 - z1;
-            s_x = 0.5 * (dyA * dzB - dzA * dyB);
-            s_y = 0.5 * (dzA * dxB - dxA * dzB);
-            s_z = 0.5 * (dxA * dyB - dyA * dxB);
-        }
-        const int L = B.face2cell[2 * f];
-        const double c0 = B.fcenter[3 * f] - B.ccenter[3 * L];
-        const double c1 = B.fcenter[3 * f + 1] - B.ccenter[3 * L + 1];
-        const double c2 = B.fcenter[3 * f + 2] - B.ccenter[3 * L + 2];
-        const double dot = c0 * s_x + c1 * s_y + c2 * s_z;
-        if (dot < 0) {
-            s_x *= -1.0;
-            s_y *= -1.0;
-            s_z *= -1.0;
-        }
-        B.S[3 * f] = s_x;
-        B.S[3 * f + 1] = s_y;
-        B.S[3 * f + 2] = s_z;
-    }
-    B.area.resize(B.nface);
-    for (int f = 0; f < B.nface; ++f) {  // computeAreaFaces :548-561
-        const double s_x = B.S[3 * f], s_y = B.S[3 * f + 1], s_z = B.S[3 * f + 2];
-        B.area[f] = std::sqrt(s_x * s_x + s_y * s_y + s_z * s_z);
-    }
-    B.volume.assign(B.nall, 0.0);
-    for (int c = 0; c < nreal; ++c) {  // computeVolumeCells :575-624
-        double volume = 0.0;
-        for (int k = B.cell_off[c]; k < B.cell_off[c + 1]; ++k) {
-            const int f = B.cell2face[k];
-            double s_x = B.S[3 * f], s_y = B.S[3 * f + 1], s_z = B.S[3 * f + 2];
-            const double cv0 = B.fcenter[3 * f] - B.ccenter[3 * c];
-            const double cv1 = B.fcenter[3 * f + 1] - B.ccenter[3 * c + 1];
-            const double cv2 = B.fcenter[3 * f + 2] - B.ccenter[3 * c + 2];
-            const double dot = cv0 * s_x + cv1 * s_y + cv2 * s_z;
+    parallel_for(B.nface, [&](size_t f0, size_t f1) {
+        for (int f = (int)f0; f < (int)f1; ++f) {  // computeNormalFaces :337-448
+            const int* fn = &B.face_nodes[4 * f];
+            double s_x, s_y, s_z;
+            if (B.face_nnode[f] == 3) {
+                const double x1 = X[3 * fn[0]], x2 = X[3 * fn[1]], x3 = X[3 * fn[2]];
+                const double y1 = X[3 * fn[0] + 1], y2 = X[3 * fn[1] + 1], y3 = X[3 * fn[2] + 1];
+                const double z1 = X[3 * fn[0] + 2], z2 = X[3 * fn[1] + 2], z3 = X[3 * fn[2] + 2];
+                const double dxyA = (x1 - x2) * (y1 + y2), dxyB = (x2 - x3) * (y2 + y3), dxyC = (x3 - x1) * (y3 + y1);
+                const double dyzA = (y1 - y2) * (z1 + z2), dyzB = (y2 - y3) * (z2 + z3), dyzC = (y3 - y1) * (z3 + z1);
+                const double dzxA = (z1 - z2) * (x1 + x2), dzxB = (z2 - z3) * (x2 + x3), dzxC = (z3 - z1) * (x3 + x1);
+                s_x = 0.5 * (dyzA + dyzB + dyzC);
+                s_y = 0.5 * (dzxA + dzxB + dzxC);
+                s_z = 0.5 * (dxyA + dxyB + dxyC);
+            } else {
+                const double x1 = X[3 * fn[0]], x2 = X[3 * fn[1]], x3 = X[3 * fn[2]], x4 = X[3 * fn[3]];
+                const double y1 = X[3 * fn[0] + 1], y2 = X[3 * fn[1] + 1], y3 = X[3 * fn[2] + 1], y4 = X[3 * fn[3] + 1];
+                const double z1 = X[3 * fn[0] + 2], z2 = X[3 * fn[1] + 2], z3 = X[3 * fn[2] + 2], z4 = X[3 * fn[3] + 2];
+                const double dxA = x4 - x2, dxB = x3 - x1, dyA = y4 - y2, dyB = y3 - y1, dzA = z4 - z2, dzB = z3 - z1;
+                s_x = 0.5 * (dyA * dzB - dzA * dyB);
+                s_y = 0.5 * (dzA * dxB - dxA * dzB);
+                s_z = 0.5 * (dxA * dyB - dyA * dxB);
+            }
+            const int L = B.face2cell[2 * f];
+            const double c0 = B.fcenter[3 * f] - B.ccenter[3 * L];
+            const double c1 = B.fcenter[3 * f + 1] - B.ccenter[3 * L + 1];
+            const double c2 = B.fcenter[3 * f + 2] - B.ccenter[3 * L + 2];
+            const double dot = c0 * s_x + c1 * s_y + c2 * s_z;
             if (dot < 0) {
                 s_x *= -1.0;
                 s_y *= -1.0;
                 s_z *= -1.0;
             }
-            volume += (B.fcenter[3 * f] * s_x + B.fcenter[3 * f + 1] * s_y + B.fcenter[3 * f + 2] * s_z) / 3;
+            B.S[3 * f] = s_x;
+            B.S[3 * f + 1] = s_y;
+            B.S[3 * f + 2] = s_z;
         }
-        B.volume[c] = volume;
-    }
+    });
+    B.area.resize(B.nface);
+    parallel_for(B.nface, [&](size_t f0, size_t f1) {
+        for (int f = (int)f0; f < (int)f1; ++f) {  // computeAreaFaces :548-561
+            const double s_x = B.S[3 * f], s_y = B.S[3 * f + 1], s_z = B.S[3 * f + 2];
+            B.area[f] = std::sqrt(s_x * s_x + s_y * s_y + s_z * s_z);
+        }
+    });
+    B.volume.assign(B.nall, 0.0);
+    parallel_for(nreal, [&](size_t c0, size_t c1) {
+        for (int c = (int)c0; c < (int)c1; ++c) {  // computeVolumeCells :575-624
+            double volume = 0.0;
+            for (int k = B.cell_off[c]; k < B.cell_off[c + 1]; ++k) {
+                const int f = B.cell2face[k];
+                double s_x = B.S[3 * f], s_y = B.S[3 * f + 1], s_z = B.S[3 * f + 2];
+                const double cv0 = B.fcenter[3 * f] - B.ccenter[3 * c];
+                const double cv1 = B.fcenter[3 * f + 1] - B.ccenter[3 * c + 1];
+                const double cv2 = B.fcenter[3 * f + 2] - B.ccenter[3 * c + 2];
+                const double dot = cv0 * s_x + cv1 * s_y + cv2 * s_z;
+                if (dot < 0) {
+                    s_x *= -1.0;
+                    s_y *= -1.0;
+                    s_z *= -1.0;
+                }
+                volume += (B.fcenter[3 * f] * s_x + B.fcenter[3 * f + 1] * s_y + B.fcenter[3 * f + 2] * s_z) / 3;
+            }
+            B.volume[c] = volume;
+        }
+    });
     for (int c = nreal; c < B.nall; ++c) B.volume[c] = B.volume[B.cell2cell[B.cell_off[c]]];  // :628-636
     B.rL.resize(3 * (size_t)B.nface);
     B.rR.resize(3 * (size_t)B.nface);
-    for (int f = 0; f < B.nface; ++f) {  // computeInterpVect :460-504
-        const int L = B.face2cell[2 * f], R = B.face2cell[2 * f + 1];
-        for (int d = 0; d < 3; ++d) {
-            B.rL[3 * f + d] = B.fcenter[3 * f + d] - B.ccenter[3 * L + d];
-            B.rR[3 * f + d] = B.fcenter[3 * f + d] - B.ccenter[3 * R + d];
+    parallel_for(B.nface, [&](size_t f0, size_t f1) {
+        for (int f = (int)f0; f < (int)f1; ++f) {  // computeInterpVect :460-504
+            const int L = B.face2cell[2 * f], R = B.face2cell[2 * f + 1];
+            for (int d = 0; d < 3; ++d) {
+                B.rL[3 * f + d] = B.fcenter[3 * f + d] - B.ccenter[3 * L + d];
+                B.rR[3 * f + d] = B.fcenter[3 * f + d] - B.ccenter[3 * R + d];
+            }
         }
-    }
+    });
     for (int c = nreal; c < B.nall; ++c) {  // :509-540 reflected right vector of ghost faces
         const int f = B.cell2face[B.cell_off[c]];
         const double r_x = B.rL[3 * f], r_y = B.rL[3 * f + 1], r_z = B.rL[3 * f + 2];
@@ -299,45 +312,47 @@ HostBlock build_block(const RawBlock& raw) {
     }
     B.lsqw.assign(3 * (size_t)B.cell_off[nreal], 0.0);
     if (nreal > 1) {  // computeWLS :673-800 (Gram-Schmidt weights)
-        std::vector<double> dx, dy, dz;
-        for (int c = 0; c < nreal; ++c) {
-            const int k0 = B.cell_off[c], nb = B.cell_off[c + 1] - k0;
-            dx.resize(nb);
-            dy.resize(nb);
-            dz.resize(nb);
-            for (int j = 0; j < nb; ++j) {
-                const int o = B.cell2cell[k0 + j];
-                dx[j] = B.ccenter[3 * o] - B.ccenter[3 * c];
-                dy[j] = B.ccenter[3 * o + 1] - B.ccenter[3 * c + 1];
-                dz[j] = B.ccenter[3 * o + 2] - B.ccenter[3 * c + 2];
+        parallel_for(nreal, [&](size_t c0, size_t c1) {
+            std::vector<double> dx, dy, dz;
+            for (int c = (int)c0; c < (int)c1; ++c) {
+                const int k0 = B.cell_off[c], nb = B.cell_off[c + 1] - k0;
+                dx.resize(nb);
+                dy.resize(nb);
+                dz.resize(nb);
+                for (int j = 0; j < nb; ++j) {
+                    const int o = B.cell2cell[k0 + j];
+                    dx[j] = B.ccenter[3 * o] - B.ccenter[3 * c];
+                    dy[j] = B.ccenter[3 * o + 1] - B.ccenter[3 * c + 1];
+                    dz[j] = B.ccenter[3 * o + 2] - B.ccenter[3 * c + 2];
+                }
+                double sxx = 0.0, sxy = 0.0, sxz = 0.0, syy = 0.0, syz = 0.0, szz = 0.0;
+                for (int j = 0; j < nb; ++j) {
+                    sxx += dx[j] * dx[j];
+                    sxy += dx[j] * dy[j];
+                    sxz += dx[j] * dz[j];
+                }
+                const double r11 = std::sqrt(sxx);
+                const double r12 = (1 / r11) * sxy;
+                const double r13 = (1 / r11) * sxz;
+                for (int j = 0; j < nb; ++j) {
+                    syy += dy[j] * dy[j];
+                    syz += dy[j] * dz[j];
+                }
+                const double r22 = std::sqrt(syy - r12 * r12);
+                const double r23 = (1 / r22) * (syz - r12 * r13);
+                for (int j = 0; j < nb; ++j) szz += dz[j] * dz[j];
+                const double r33 = std::sqrt(szz - (r13 * r13 + r23 * r23));
+                const double beta = (r12 * r23 - r13 * r22) / (r11 * r22);
+                for (int j = 0; j < nb; ++j) {
+                    const double alpha3 = (1 / (r33 * r33)) * (dz[j] - (r23 / r22) * dy[j] + beta * dx[j]);
+                    const double alpha2 = (1 / (r22 * r22)) * (dy[j] - (r12 / r11) * dx[j]);
+                    const double alpha1 = dx[j] / (r11 * r11);
+                    B.lsqw[3 * (k0 + j)] = alpha1 - (r12 / r11) * alpha2 + beta * alpha3;
+                    B.lsqw[3 * (k0 + j) + 1] = alpha2 - (r23 / r22) * alpha3;
+                    B.lsqw[3 * (k0 + j) + 2] = alpha3;
+                }
             }
-            double sxx = 0.0, sxy = 0.0, sxz = 0.0, syy = 0.0, syz = 0.0, szz = 0.0;
-            for (int j = 0; j < nb; ++j) {
-                sxx += dx[j] * dx[j];
-                sxy += dx[j] * dy[j];
-                sxz += dx[j] * dz[j];
-            }
-            const double r11 = std::sqrt(sxx);
-            const double r12 = (1 / r11) * sxy;
-            const double r13 = (1 / r11) * sxz;
-            for (int j = 0; j < nb; ++j) {
-                syy += dy[j] * dy[j];
-                syz += dy[j] * dz[j];
-            }
-            const double r22 = std::sqrt(syy - r12 * r12);
-            const double r23 = (1 / r22) * (syz - r12 * r13);
-            for (int j = 0; j < nb; ++j) szz += dz[j] * dz[j];
-            const double r33 = std::sqrt(szz - (r13 * r13 + r23 * r23));
-            const double beta = (r12 * r23 - r13 * r22) / (r11 * r22);
-            for (int j = 0; j < nb; ++j) {
-                const double alpha3 = (1 / (r33 * r33)) * (dz[j] - (r23 / r22) * dy[j] + beta * dx[j]);
-                const double alpha2 = (1 / (r22 * r22)) * (dy[j] - (r12 / r11) * dx[j]);
-                const double alpha1 = dx[j] / (r11 * r11);
-                B.lsqw[3 * (k0 + j)] = alpha1 - (r12 / r11) * alpha2 + beta * alpha3;
-                B.lsqw[3 * (k0 + j) + 1] = alpha2 - (r23 / r22) * alpha3;
-                B.lsqw[3 * (k0 + j) + 2] = alpha3;
-            }
-        }
+        });
     }
     return B;
 }

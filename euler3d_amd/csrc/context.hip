@@ -14,6 +14,7 @@
 #include "device.hpp"
 #include "mesh_internal.hpp"
 #include "ordering.hpp"
+#include "parallel.hpp"
 #include "physics.hpp"
 
 namespace e3d {
@@ -481,56 +482,70 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         tile0[b] = c->roff[b] / kWG;
         ntile[b] = (B.nreal + kWG - 1) / kWG;
         for (int t = 0; t < ntile[b]; ++t) wg_block[tile0[b] + t] = b;
-        for (int f = 0; f < B.nface; ++f) {
-            const size_t g = global_face(c, b, f);
-            for (int d = 0; d < 3; ++d) {
-                S[d * (size_t)NF + g] = B.S[3 * f + d];
-                rL[d * (size_t)NF + g] = B.rL[3 * f + d];
-                rR[d * (size_t)NF + g] = B.rR[3 * f + d];
-                fc[d * (size_t)NF + g] = B.fcenter[3 * f + d];
-            }
-            area[g] = B.area[f];
-        }
-        for (int q = 0; q < B.nreal; ++q) {
-            const int gc = global_cell(c, b, q);
-            nf[gc] = B.cell_off[q + 1] - B.cell_off[q];
-            vol[gc] = B.volume[q];
-            for (int k = 0; k < nf[gc]; ++k) {
-                const int f = B.cell2face[B.cell_off[q] + k];
-                const bool right = B.face2cell[2 * f] != q;
-                const int other = B.cell2cell[B.cell_off[q] + k];
-                nbr[(size_t)k * NR + gc] = global_cell(c, b, other);
-                fid[(size_t)k * NR + gc] = global_face(c, b, f) | (right ? (int)0x80000000u : 0);
-                for (int d = 0; d < 3; ++d)
-                    lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
-                // geometry copies: S, own-side r, S/|S| and |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz)
-                // as in FluxScheme.cpp:105-112 (equal to the stored face area)
-                const double sx = B.S[3 * f], sy = B.S[3 * f + 1], sz = B.S[3 * f + 2];
-                const double norm = std::sqrt(sx * sx + sy * sy + sz * sz);
-                const double* rv = right ? &B.rR[3 * f] : &B.rL[3 * f];
-                const double sv[3] = {sx, sy, sz};
+        parallel_for(B.nface, [&](size_t f0, size_t f1) {
+            for (int f = (int)f0; f < (int)f1; ++f) {
+                const size_t g = global_face(c, b, f);
                 for (int d = 0; d < 3; ++d) {
-                    // oriented away from this cell, as the gradient uses it (GreenGauss.cpp:104-118)
-                    sSR[pair_at(d, k, gc)] = right ? sv[d] * -1.0 : sv[d];
-                    sSR[pair_at(3 + d, k, gc)] = rv[d];
-                    sNA[pair_at(d, k, gc)] = sv[d] / norm;
+                    S[d * (size_t)NF + g] = B.S[3 * f + d];
+                    rL[d * (size_t)NF + g] = B.rL[3 * f + d];
+                    rR[d * (size_t)NF + g] = B.rR[3 * f + d];
+                    fc[d * (size_t)NF + g] = B.fcenter[3 * f + d];
                 }
-                sNA[pair_at(3, k, gc)] = norm;
-                // face-state record of the other cell on this face
-                size_t rec;
-                if (other < B.nreal) {
-                    int ko = -1;
-                    for (int j = B.cell_off[other]; j < B.cell_off[other + 1]; ++j)
-                        if (B.cell2face[j] == f) ko = j - B.cell_off[other];
-                    if (ko < 0) return set_error(c, E3D_EINVAL, "upload: inconsistent face connectivity");
-                    rec = (size_t)ko * NR + (size_t)global_cell(c, b, other);
-                } else {
-                    rec = ghost_record(nslot, (int)NR, c->goff[b] + (other - B.nreal));
-                }
-                if (rec >= 0x80000000ull) return set_error(c, E3D_EINVAL, "mesh too large for face-state records");
-                mate[(size_t)k * NR + gc] = (int)rec | (right ? (int)0x80000000u : 0);
+                area[g] = B.area[f];
             }
-        }
+        });
+        // per cell and face slot (cells write disjoint entries; several host threads)
+        std::atomic<int> bad{0};
+        parallel_for(B.nreal, [&](size_t q0, size_t q1) {
+            for (int q = (int)q0; q < (int)q1; ++q) {
+                const int gc = global_cell(c, b, q);
+                nf[gc] = B.cell_off[q + 1] - B.cell_off[q];
+                vol[gc] = B.volume[q];
+                for (int k = 0; k < nf[gc]; ++k) {
+                    const int f = B.cell2face[B.cell_off[q] + k];
+                    const bool right = B.face2cell[2 * f] != q;
+                    const int other = B.cell2cell[B.cell_off[q] + k];
+                    nbr[(size_t)k * NR + gc] = global_cell(c, b, other);
+                    fid[(size_t)k * NR + gc] = global_face(c, b, f) | (right ? (int)0x80000000u : 0);
+                    for (int d = 0; d < 3; ++d)
+                        lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
+                    // geometry copies: S, own-side r, S/|S| and |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz)
+                    // as in FluxScheme.cpp:105-112 (equal to the stored face area)
+                    const double sx = B.S[3 * f], sy = B.S[3 * f + 1], sz = B.S[3 * f + 2];
+                    const double norm = std::sqrt(sx * sx + sy * sy + sz * sz);
+                    const double* rv = right ? &B.rR[3 * f] : &B.rL[3 * f];
+                    const double sv[3] = {sx, sy, sz};
+                    for (int d = 0; d < 3; ++d) {
+                        // oriented away from this cell, as the gradient uses it (GreenGauss.cpp:104-118)
+                        sSR[pair_at(d, k, gc)] = right ? sv[d] * -1.0 : sv[d];
+                        sSR[pair_at(3 + d, k, gc)] = rv[d];
+                        sNA[pair_at(d, k, gc)] = sv[d] / norm;
+                    }
+                    sNA[pair_at(3, k, gc)] = norm;
+                    // face-state record of the other cell on this face
+                    size_t rec;
+                    if (other < B.nreal) {
+                        int ko = -1;
+                        for (int j = B.cell_off[other]; j < B.cell_off[other + 1]; ++j)
+                            if (B.cell2face[j] == f) ko = j - B.cell_off[other];
+                        if (ko < 0) {
+                            bad |= 1;
+                            continue;
+                        }
+                        rec = (size_t)ko * NR + (size_t)global_cell(c, b, other);
+                    } else {
+                        rec = ghost_record(nslot, (int)NR, c->goff[b] + (other - B.nreal));
+                    }
+                    if (rec >= 0x80000000ull) {
+                        bad |= 2;
+                        continue;
+                    }
+                    mate[(size_t)k * NR + gc] = (int)rec | (right ? (int)0x80000000u : 0);
+                }
+            }
+        });
+        if (bad & 1) return set_error(c, E3D_EINVAL, "upload: inconsistent face connectivity");
+        if (bad & 2) return set_error(c, E3D_EINVAL, "mesh too large for face-state records");
         for (size_t j = 0; j < B.bnds.size(); ++j) {
             const bool real_bnd = (int)j < B.n_real_boundaries;
             const BcType t = B.bnds[j].type;
