@@ -272,13 +272,15 @@ def main():
     ctx.set_params(params)
     ctx.set_ordering({"reference": E.ORDER_REFERENCE, "rcm": E.ORDER_RCM, "morton": E.ORDER_MORTON}[a.order])
     print(f"[bench] mesh built in {t_mesh:.1f}s; uploading", file=sys.stderr, flush=True)
+    t_up = time.perf_counter()
     if a.graph:
         ctx.upload_batch([mesh], [dict(mach=sc["mach"], aoa_deg=sc["aoa"], criterion=-1e9, max_iter=10 ** 9)])
     else:
         ctx.upload(mesh)
     ctx.init_freestream()
-    print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s); "
-          f"warmup {a.warmup}", file=sys.stderr, flush=True)
+    t_up = time.perf_counter() - t_up
+    print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s, uploaded "
+          f"in {t_up:.1f}s); warmup {a.warmup}", file=sys.stderr, flush=True)
     if a.warmup:
         if a.graph:
             ctx.batch_run(a.warmup, use_graph=True)
