@@ -40,13 +40,44 @@ template <class T>
 __device__ __forceinline__ void st32(T* base, int i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)i * (unsigned)sizeof(T)) = v;
 }
+// Streaming (nontemporal) forms of ld32/st32, for data a launch touches once
+// in a coalesced stream: the per-slot geometry k_interp reads, and the
+// face-state records, residuals, W0, dt and new state the kernels write.
+// They leave the caches to the gathered neighbour states and records.
+// Measured on the 4.7M-cell channel in 8 blocks: 5.14 -> 4.98 ms per
+// iteration. Streaming loads of the records in k_flux and streaming stores of
+// the scattered ghost values measured slower and are not used.
+typedef double double2_ev __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ void st32_stream(T* base, int i, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)i * (unsigned)sizeof(T)));
+}
+__device__ __forceinline__ void st32_stream(double2* base, int i, double2 v) {
+    double2_ev w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<double2_ev*>(reinterpret_cast<char*>(base) + (unsigned)i * 16u));
+}
+template <class T>
+__device__ __forceinline__ T ld32_stream(const T* base, int i) {
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)i * (unsigned)sizeof(T)));
+}
+__device__ __forceinline__ double2 ld32_stream(const double2* base, int i) {
+    const double2_ev v = __builtin_nontemporal_load(
+        reinterpret_cast<const double2_ev*>(reinterpret_cast<const char*>(base) + (unsigned)i * 16u));
+    return make_double2(v.x, v.y);
+}
 __device__ __forceinline__ void store_w32(double* W, int NA, int x, const double w[5]) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) st32(W + (size_t)v * NA, x, w[v]);
+    for (int v = 0; v < 5; ++v) st32_stream(W + (size_t)v * NA, x, w[v]);
 }
 // Pair p of slot k of cell c in a per-slot pair array.
 __device__ __forceinline__ double2 pair32(const double2* a, int p, int nslot, int NRp, int k, int c) {
     return ld32(a + ((size_t)p * nslot + k) * NRp, c);
+}
+__device__ __forceinline__ double2 pair32_stream(const double2* a, int p, int nslot, int NRp, int k, int c) {
+    return ld32_stream(a + ((size_t)p * nslot + k) * NRp, c);
 }
 // Unit normal and |S| of slot k of cell c.
 __device__ __forceinline__ double slot_normal(const DMesh& m, int k, int c, double n[3]) {
@@ -102,9 +133,9 @@ __device__ __forceinline__ void load_own_rec(const double* Wf, size_t nrec, int 
 }
 __device__ __forceinline__ void store_own_rec(double* Wf, size_t nrec, int NRp, int k, int c, const double w[5]) {
     const size_t r0 = (size_t)k * NRp;
-    st32(reinterpret_cast<double2*>(Wf) + r0, c, make_double2(w[0], w[1]));
-    st32(reinterpret_cast<double2*>(Wf + 2 * nrec) + r0, c, make_double2(w[2], w[3]));
-    st32(Wf + 4 * nrec + r0, c, w[4]);
+    st32_stream(reinterpret_cast<double2*>(Wf) + r0, c, make_double2(w[0], w[1]));
+    st32_stream(reinterpret_cast<double2*>(Wf + 2 * nrec) + r0, c, make_double2(w[2], w[3]));
+    st32_stream(Wf + 4 * nrec + r0, c, w[4]);
 }
 
 // W + (lim*grad).r, accumulated per component as FluxScheme.cpp:127-134 does.
@@ -194,15 +225,15 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
             load_w32(s.W, m.NA, nbv[k], wn);
             // S oriented away from this cell (Green-Gauss) or the least-squares
             // weights; the r vector goes to LDS for the limiter and reconstruction
-            const double2 p1 = pair32(m.sSR, 1, ns, NRp, k, c), p2 = pair32(m.sSR, 2, ns, NRp, k, c);
+            const double2 p1 = pair32_stream(m.sSR, 1, ns, NRp, k, c), p2 = pair32_stream(m.sSR, 2, ns, NRp, k, c);
             if (GRAD == 0) {
-                const double2 p0 = pair32(m.sSR, 0, ns, NRp, k, c);
+                const double2 p0 = pair32_stream(m.sSR, 0, ns, NRp, k, c);
                 geo[0] = p0.x;
                 geo[1] = p0.y;
                 geo[2] = p1.x;
             } else {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) geo[d] = ld32(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
+                for (int d = 0; d < 3; ++d) geo[d] = ld32_stream(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
             }
             r_lds[k * 3 + 0][threadIdx.x] = p1.y;
             r_lds[k * 3 + 1][threadIdx.x] = p2.x;
@@ -341,11 +372,11 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             }
         if (DISS) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) st32(s.diss + (size_t)v * NRp, c, ds[v]);
+            for (int v = 0; v < 5; ++v) st32_stream(s.diss + (size_t)v * NRp, c, ds[v]);
         }
         if (LAST) {  // the last stage's convective residual feeds the monitors and the array getters
 #pragma unroll
-            for (int v = 0; v < 5; ++v) st32(s.conv + (size_t)v * NRp, c, cv[v]);
+            for (int v = 0; v < 5; ++v) st32_stream(s.conv + (size_t)v * NRp, c, cv[v]);
         }
         double w0[5], wnew[5], dt;
         const double vol = ld32(m.vol, c);
@@ -357,8 +388,8 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             w0[4] = 0.5 * wc[0] * (wc[1] * wc[1] + wc[2] * wc[2] + wc[3] * wc[3]) + inv_gm1 * wc[4];
             dt = cfl * vol / sr;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) st32(s.W0 + (size_t)v * NRp, c, w0[v]);
-            st32(s.dt, c, dt);
+            for (int v = 0; v < 5; ++v) st32_stream(s.W0 + (size_t)v * NRp, c, w0[v]);
+            st32_stream(s.dt, c, dt);
         } else {
 #pragma unroll
             for (int v = 0; v < 5; ++v) w0[v] = ld32(s.W0 + (size_t)v * NRp, c);
