@@ -41,12 +41,14 @@ __device__ __forceinline__ void st32(T* base, int i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)i * (unsigned)sizeof(T)) = v;
 }
 // Streaming (nontemporal) forms of ld32/st32, for data a launch touches once
-// in a coalesced stream: the per-slot geometry k_interp reads, and the
+// in a coalesced stream: per-cell and per-slot inputs (face count, volume,
+// neighbour and record ids, slot geometry, W0, dt, dissipation) and the
 // face-state records, residuals, W0, dt and new state the kernels write.
 // They leave the caches to the gathered neighbour states and records.
-// Measured on the 4.7M-cell channel in 8 blocks: 5.14 -> 4.98 ms per
-// iteration. Streaming loads of the records in k_flux and streaming stores of
-// the scattered ghost values measured slower and are not used.
+// Measured on the 4.7M-cell channel in 8 blocks: 5.14 -> 4.94 ms per
+// iteration. Streaming loads of the records in k_flux (each is read again as
+// the other side's record) and streaming stores of the scattered ghost values
+// measured slower and are not used.
 typedef double double2_ev __attribute__((ext_vector_type(2)));
 template <class T>
 __device__ __forceinline__ void st32_stream(T* base, int i, T v) {
@@ -81,7 +83,7 @@ __device__ __forceinline__ double2 pair32_stream(const double2* a, int p, int ns
 }
 // Unit normal and |S| of slot k of cell c.
 __device__ __forceinline__ double slot_normal(const DMesh& m, int k, int c, double n[3]) {
-    const double2 a = pair32(m.sNA, 0, m.nslot, m.NRp, k, c), b = pair32(m.sNA, 1, m.nslot, m.NRp, k, c);
+    const double2 a = pair32_stream(m.sNA, 0, m.nslot, m.NRp, k, c), b = pair32_stream(m.sNA, 1, m.nslot, m.NRp, k, c);
     n[0] = a.x;
     n[1] = a.y;
     n[2] = b.x;
@@ -198,12 +200,12 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     const int tile = tile_of(m, ntiles);
     if (tile >= ntiles || !instance_active(m, tile_instance(m, tile))) return;
     const int c = tile * kWG + threadIdx.x;
-    const int nf = ld32(m.nf, c);
+    const int nf = ld32_stream(m.nf, c);
     if (nf == 0) return;
     const int NRp = m.NRp, ns = m.nslot;
     double wc[5];
     load_w32(s.W, m.NA, c, wc);
-    const double vol = ld32(m.vol, c);
+    const double vol = ld32_stream(m.vol, c);
     // pass 1: gradient and the neighbour extrema, faces in the reference's order.
     // The neighbour ids of all slots are loaded first (slots k >= nf hold
     // in-bounds padding and are not used), so each face then waits for one
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     unsigned ghost_slots = 0;  // bit k: the neighbour across face slot k is a ghost
     int nbv[MAXS];
 #pragma unroll
-    for (int k = 0; k < MAXS; ++k) nbv[k] = k < ns ? ld32(m.nbr + (size_t)k * NRp, c) : c;
+    for (int k = 0; k < MAXS; ++k) nbv[k] = k < ns ? ld32_stream(m.nbr + (size_t)k * NRp, c) : c;
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
 #pragma unroll
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
     if (!instance_active(m, inst)) return;
     const double* fs = m.inst_fs + 5 * inst;
     const int c = tile * kWG + threadIdx.x;
-    const int nf = ld32(m.nf, c);
+    const int nf = ld32_stream(m.nf, c);
     const int NRp = m.NRp, ns = m.nslot;
     const size_t ghost0 = (size_t)ns * NRp;
     double res2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -322,12 +324,12 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
         if (FIRST) c_center = sqrt(gas.gamma * wc[4] / wc[0]);
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-            ds[v] = DISS ? ld32(s.diss + (size_t)v * NRp, c) * (1.0 - beta) : ld32(s.diss + (size_t)v * NRp, c);
+            ds[v] = DISS ? ld32_stream(s.diss + (size_t)v * NRp, c) * (1.0 - beta) : ld32_stream(s.diss + (size_t)v * NRp, c);
         // the other side's record index of every slot first (padding slots
         // hold 0, in bounds), so each face then waits for one round trip
         int mtv[MAXS];
 #pragma unroll
-        for (int k = 0; k < MAXS; ++k) mtv[k] = k < ns ? ld32(m.mate + (size_t)k * NRp, c) : 0;
+        for (int k = 0; k < MAXS; ++k) mtv[k] = k < ns ? ld32_stream(m.mate + (size_t)k * NRp, c) : 0;
 #pragma unroll
         for (int k = 0; k < MAXS; ++k)
             if (k < nf) {
@@ -379,7 +381,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             for (int v = 0; v < 5; ++v) st32_stream(s.conv + (size_t)v * NRp, c, cv[v]);
         }
         double w0[5], wnew[5], dt;
-        const double vol = ld32(m.vol, c);
+        const double vol = ld32_stream(m.vol, c);
         if (FIRST) {
             w0[0] = wc[0];
             w0[1] = wc[0] * wc[1];
@@ -392,8 +394,8 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
             st32_stream(s.dt, c, dt);
         } else {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) w0[v] = ld32(s.W0 + (size_t)v * NRp, c);
-            dt = ld32(s.dt, c);
+            for (int v = 0; v < 5; ++v) w0[v] = ld32_stream(s.W0 + (size_t)v * NRp, c);
+            dt = ld32_stream(s.dt, c);
         }
         rk_update(gas.gm1, alpha * dt / vol, w0, cv, ds, wnew);
         store_w32(s.Wn, m.NA, c, wnew);
