@@ -322,6 +322,9 @@ int sync_instances(e3d_ctx* c) {
         const e3d_instance& q = c->inst[c->blk_inst[b]];
         qcmac[b] = 0.5 * p.gamma * q.mach * q.mach * q.cmac;
     }
+    // the context's stream does not synchronise with the null stream: let the
+    // work queued on it (which reads these constants) finish first
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(c->d_inst_fs, c->inst_fs.data(), sizeof(double) * c->inst_fs.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_blk_qcmac, qcmac.data(), sizeof(double) * qcmac.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_prm, prm.data(), sizeof(InstParam) * prm.size(), hipMemcpyHostToDevice));
@@ -1159,6 +1162,7 @@ int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
     if (!c->batch || i < 0 || i >= c->n_inst || !rows || max_rows < 0)
         return set_error(c, E3D_EINVAL, "e3d_batch_history: bad argument");
     int it = 0;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));  // rows written by work still queued on the stream
     HIP_TRY(c, hipMemcpy(&it, c->d_iter + i, sizeof(int), hipMemcpyDeviceToHost));
     const int n = std::min(std::min(it, c->hist_cap), max_rows);
     if (n > 0)

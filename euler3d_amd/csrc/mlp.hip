@@ -412,7 +412,11 @@ int e3d_mlp_create(int device, int n_in, int width, int depth, int n_out, unsign
             e3d_mlp_destroy(m);
             return rc;
         }
-        e = hipMemcpy(W, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice);
+        // on the network's stream, after the zero fill alloc() queued there (the
+        // stream does not synchronise with the null stream), and complete before
+        // the host vector goes away
+        e = hipMemcpyAsync(W, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
         if (e != hipSuccess) {
             e3d_mlp_destroy(m);
             return e3d::set_error(nullptr, E3D_EHIP, std::string("e3d_mlp_create: ") + hipGetErrorString(e));
@@ -634,7 +638,7 @@ int e3d_mlp_time_step(e3d_mlp* m, int B, int reps, double* ms, double* flops) {
     if ((rc = ensure_work(m, B))) return rc;
     std::vector<int> idx(B);
     for (int b = 0; b < B; ++b) idx[b] = (int)(b % m->n_data);
-    MLP_TRY(m, hipMemcpy(m->idx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice));
+    MLP_TRY(m, hipMemcpyAsync(m->idx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
     hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
                        m->n_out, B, m->X0, m->X0t, m->Tb);
     hipEvent_t a, b;
