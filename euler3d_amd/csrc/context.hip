@@ -102,7 +102,7 @@ struct e3d_ctx {
     hipGraphExec_t gexec = nullptr;
     double* graph_W = nullptr;   // W buffer the graph was captured with
     bool capturing = false;
-    int tile_map = 1;            // tile order of the stage kernels (e3d_set_tile_order)
+    int tile_map = 0;            // tile order of the stage kernels (e3d_set_tile_order)
     int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
     // per block: device position of each real cell and face (empty: the reference's numbering)
     std::vector<std::vector<int>> cell_pos, face_pos;
