@@ -20,6 +20,7 @@ GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
 LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1
 ARR_W, ARR_W0, ARR_CONV, ARR_DISS, ARR_GRAD, ARR_LIM, ARR_DT, ARR_SPECRAD = range(8)
 ORDER_REFERENCE, ORDER_RCM, ORDER_MORTON = 0, 1, 2
+TILES_LINEAR, TILES_XCD_CONTIGUOUS, TILES_XCD_CHUNKS = 0, 1, 2
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
@@ -323,8 +324,9 @@ class Context:
         n = self._c(lib().e3d_batch_history(self._h, i, rows, max_rows))
         return [rows[k].as_dict() for k in range(n)]
 
-    def set_tile_order(self, xcd_contiguous):
-        self._c(lib().e3d_set_tile_order(self._h, 1 if xcd_contiguous else 0))
+    def set_tile_order(self, order):
+        """TILES_LINEAR, TILES_XCD_CONTIGUOUS or TILES_XCD_CHUNKS (default); addresses only."""
+        self._c(lib().e3d_set_tile_order(self._h, int(order)))
 
     def set_ordering(self, mode):
         """Memory order of the next upload: ORDER_REFERENCE, ORDER_RCM or ORDER_MORTON."""

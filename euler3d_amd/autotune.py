@@ -10,8 +10,9 @@ Three knobs change how the fused stage kernels walk memory:
 * the device memory order of cells and faces (``e3d_set_ordering``: the
   file's numbering, reverse Cuthill-McKee or a Morton curve), which changes
   addresses only: every result bit stays the same;
-* the workgroup-to-tile order: eight contiguous runs of tiles, one per XCD
-  (each XCD has its own L2), or tile i on workgroup i; addresses only.
+* the workgroup-to-tile order: tile i on workgroup i, one contiguous run of
+  tiles per XCD (each XCD has its own L2), or runs of 16 tiles dealt to the
+  XCDs in turn; addresses only.
 
 ``tune(nx, ny, nz, scheme)`` times every combination on the GPU (a few RK
 iterations through ``Context.iterate``) and keeps the fastest; results are
@@ -24,11 +25,11 @@ import json
 import os
 import time
 
-from . import (FLUX_ROE, GRAD_GREEN_GAUSS, LIM_VENKATAKRISHNAN, ORDER_MORTON, ORDER_RCM, ORDER_REFERENCE, Context,
-               Mesh, lib, make_params)
+from . import (FLUX_ROE, GRAD_GREEN_GAUSS, LIM_VENKATAKRISHNAN, ORDER_MORTON, ORDER_RCM, ORDER_REFERENCE,
+               TILES_LINEAR, TILES_XCD_CHUNKS, TILES_XCD_CONTIGUOUS, Context, Mesh, lib, make_params)
 
 BRICKS = (0, 2, 4, 8)
-TILE_ORDERS = (1, 0)
+TILE_ORDERS = (TILES_XCD_CHUNKS, TILES_XCD_CONTIGUOUS, TILES_LINEAR)
 MEMORY_ORDERS = (ORDER_REFERENCE, ORDER_MORTON, ORDER_RCM)
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 

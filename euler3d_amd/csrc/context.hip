@@ -102,7 +102,7 @@ struct e3d_ctx {
     hipGraphExec_t gexec = nullptr;
     double* graph_W = nullptr;   // W buffer the graph was captured with
     bool capturing = false;
-    int tile_map = 0;            // tile order of the stage kernels (e3d_set_tile_order)
+    int tile_map = E3D_TILES_XCD_CHUNKS;  // tile order of the stage kernels (e3d_set_tile_order)
     int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
     // per block: device position of each real cell and face (empty: the reference's numbering)
     std::vector<std::vector<int>> cell_pos, face_pos;
@@ -1171,6 +1171,8 @@ int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
     return n;
 }
 
+static_assert(E3D_TILES_LINEAR == 0 && E3D_TILES_XCD_CONTIGUOUS == 1 && E3D_TILES_XCD_CHUNKS == 2,
+              "tile orders as tile_of numbers them");
 static_assert(E3D_ORDER_REFERENCE == ORDER_REFERENCE && E3D_ORDER_RCM == ORDER_RCM &&
                   E3D_ORDER_MORTON == ORDER_MORTON,
               "ordering modes");
@@ -1182,9 +1184,11 @@ int e3d_set_ordering(e3d_ctx* c, int mode) {
     return E3D_OK;
 }
 
-int e3d_set_tile_order(e3d_ctx* c, int xcd_contiguous) {
+int e3d_set_tile_order(e3d_ctx* c, int order) {
     if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
-    c->tile_map = c->m.tile_map = xcd_contiguous ? 1 : 0;
+    if (order < E3D_TILES_LINEAR || order > E3D_TILES_XCD_CHUNKS)
+        return set_error(c, E3D_EINVAL, "e3d_set_tile_order: order must be 0, 1 or 2");
+    c->tile_map = c->m.tile_map = order;
     drop_graph(c);
     return E3D_OK;
 }

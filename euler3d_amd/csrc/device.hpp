@@ -47,7 +47,7 @@ struct DMesh {
     const int* blk_inst;  // [nb] instance of each block
     const double* inst_fs;  // [n_inst][5] freestream primitive state of each instance
     const int* active;    // [n_inst] nonzero while the instance iterates; null: every instance iterates
-    int tile_map;         // 1: XCD-contiguous tile order (tile = (b & 7) * per + (b >> 3)), 0: tile = workgroup
+    int tile_map;         // tile order (tile_of): 0 linear, 1 XCD-contiguous, 2 XCD chunks
 };
 
 // Face-state records of the fused path: record k*NRp + c holds the

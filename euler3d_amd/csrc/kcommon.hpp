@@ -6,19 +6,38 @@
 
 namespace e3d {
 
-// Tile of kWG cells handled by this workgroup. Workgroups are dispatched
-// round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so that
-// neighbour gathers of adjacent tiles hit the same L2.
+// Tile of kWG cells handled by this workgroup, under one of three orders
+// (e3d_set_tile_order). Workgroups are dispatched round-robin over the 8 XCDs
+// (each with its own L2), so workgroup b runs on XCD b & 7 as that XCD's
+// (b >> 3)-th workgroup.
+//   0 linear:          tile = b; neighbouring tiles run on different XCDs.
+//   1 XCD-contiguous:  XCD x walks tiles [x * per, (x + 1) * per).
+//   2 XCD chunks:      runs of kTileChunk consecutive tiles dealt to the XCDs
+//                      in turn: gathers within a run stay in one L2 while the
+//                      whole chip moves through one window of memory.
+// The grid (xcd_grid) is a multiple of 8 * kTileChunk; every order maps the
+// grid one-to-one onto [0, grid), and workgroups whose tile is >= ntiles do
+// nothing.
+constexpr int kTileChunk = 16;
 __device__ __forceinline__ int xcd_tile(int ntiles) {
     const int b = blockIdx.x;
     const int per = (ntiles + 7) >> 3;
+    if ((b >> 3) >= per) return ntiles;  // grid padding beyond 8 * per
     return (b & 7) * per + (b >> 3);
 }
-inline int xcd_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+__device__ __forceinline__ int chunk_tile() {
+    const int b = blockIdx.x;
+    const int x = b & 7, w = b >> 3;
+    return ((w / kTileChunk) * 8 + x) * kTileChunk + (w % kTileChunk);
+}
+inline int xcd_grid(int ntiles) {
+    const int g = 8 * kTileChunk;
+    return g * ((ntiles + g - 1) / g);
+}
 
 // Tile of this workgroup under the mesh's tile order.
 __device__ __forceinline__ int tile_of(const DMesh& m, int ntiles) {
-    return m.tile_map ? xcd_tile(ntiles) : (int)blockIdx.x;
+    return m.tile_map == 2 ? chunk_tile() : m.tile_map == 1 ? xcd_tile(ntiles) : (int)blockIdx.x;
 }
 
 // Instance of a tile, and whether it still iterates.

@@ -48,7 +48,7 @@ def instances(n=64, seed0=1000, scale=1.0):
 def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune=True, seed0=1000, log=None):
     from . import autotune as at
     cases = instances(n, seed0, scale)
-    tile_votes = {0: 0.0, 1: 0.0}
+    tile_votes = {0: 0.0, 1: 0.0, 2: 0.0}
     mem_votes = {}
     t0 = time.perf_counter()
     for c in cases:

@@ -254,10 +254,16 @@ int e3d_batch_run(e3d_ctx* ctx, int max_iterations, int use_graph, e3d_batch_sta
  * recorded); returns n. Rows beyond the history capacity (max over the
  * instances' max_iter, at most 200000) are not kept. */
 int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
-/* Workgroup-to-tile order of the stage kernels: 1 gives each of the eight XCDs
- * a contiguous range of cell tiles, 0 (default) maps workgroup i to tile i
- * (measured faster on large meshes). Addresses only: results are the same. */
-int e3d_set_tile_order(e3d_ctx* ctx, int xcd_contiguous);
+/* Workgroup-to-tile order of the stage kernels (no counterpart in the
+ * reference, which runs one cell loop on one core). Workgroups are dealt to
+ * the GPU's eight XCDs in turn; E3D_TILES_LINEAR maps workgroup i to tile i,
+ * E3D_TILES_XCD_CONTIGUOUS gives each XCD one contiguous range of tiles,
+ * E3D_TILES_XCD_CHUNKS (default) deals runs of 16 consecutive tiles to the
+ * XCDs in turn. Addresses only: results are the same. */
+#define E3D_TILES_LINEAR 0
+#define E3D_TILES_XCD_CONTIGUOUS 1
+#define E3D_TILES_XCD_CHUNKS 2
+int e3d_set_tile_order(e3d_ctx* ctx, int order);
 
 /* Memory order of the cells and faces of the next uploaded mesh or batch.
  * E3D_ORDER_REFERENCE (default) keeps the reference's numbering

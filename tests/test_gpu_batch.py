@@ -116,14 +116,19 @@ def test_batch_tile_order_does_not_change_results():
     insts = [dict(mach=m, aoa_deg=a, criterion=-15.0, max_iter=20) for _, m, a in CASES[:2]]
     a = batch_context(meshes, insts)
     b = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
-    b.set_tile_order(False)
-    a.batch_run(8, use_graph=True)
-    b.batch_run(8, use_graph=True)
-    for blk in range(len(a.sizes)):
-        assert (a.get(blk, E.ARR_W).view(np.int64) == b.get(blk, E.ARR_W).view(np.int64)).all()
-    assert a.batch_history(1, 20) == b.batch_history(1, 20)
-    a.close()
-    b.close()
+    b.set_tile_order(E.TILES_LINEAR)
+    c = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
+    c.set_tile_order(E.TILES_XCD_CONTIGUOUS)
+    for x in (a, b, c):
+        x.batch_run(8, use_graph=True)
+    for other in (b, c):
+        for blk in range(len(a.sizes)):
+            assert (a.get(blk, E.ARR_W).view(np.int64) == other.get(blk, E.ARR_W).view(np.int64)).all()
+        assert a.batch_history(1, 20) == other.batch_history(1, 20)
+    with pytest.raises(E.E3DError):
+        a.set_tile_order(3)
+    for x in (a, b, c):
+        x.close()
 
 
 def test_batch_api_errors():
