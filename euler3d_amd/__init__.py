@@ -20,7 +20,7 @@ GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
 LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1
 ARR_W, ARR_W0, ARR_CONV, ARR_DISS, ARR_GRAD, ARR_LIM, ARR_DT, ARR_SPECRAD = range(8)
 ORDER_REFERENCE, ORDER_RCM, ORDER_MORTON = 0, 1, 2
-TILES_LINEAR, TILES_XCD_CONTIGUOUS, TILES_XCD_CHUNKS = 0, 1, 2
+TILES_AUTO, TILES_LINEAR, TILES_XCD_CONTIGUOUS, TILES_XCD_CHUNKS = -1, 0, 1, 2
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
@@ -325,7 +325,7 @@ class Context:
         return [rows[k].as_dict() for k in range(n)]
 
     def set_tile_order(self, order):
-        """TILES_LINEAR, TILES_XCD_CONTIGUOUS or TILES_XCD_CHUNKS (default); addresses only."""
+        """TILES_AUTO (default), TILES_LINEAR, TILES_XCD_CONTIGUOUS or TILES_XCD_CHUNKS; addresses only."""
         self._c(lib().e3d_set_tile_order(self._h, int(order)))
 
     def set_ordering(self, mode):

@@ -102,7 +102,8 @@ struct e3d_ctx {
     hipGraphExec_t gexec = nullptr;
     double* graph_W = nullptr;   // W buffer the graph was captured with
     bool capturing = false;
-    int tile_map = E3D_TILES_XCD_CHUNKS;  // tile order of the stage kernels (e3d_set_tile_order)
+    int tile_pref = E3D_TILES_AUTO;       // tile order asked for (e3d_set_tile_order)
+    int tile_map = E3D_TILES_XCD_CHUNKS;  // tile order in use (resolve_tile_order)
     int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
     // per block: device position of each real cell and face (empty: the reference's numbering)
     std::vector<std::vector<int>> cell_pos, face_pos;
@@ -143,6 +144,19 @@ void drop_graph(e3d_ctx* c) {
     c->gexec = nullptr;
     c->graph = nullptr;
     c->graph_W = nullptr;
+}
+
+// The tile order in use: the one asked for, or by size (E3D_TILES_AUTO): a
+// mesh of fewer than kAutoChunkTiles tiles keeps each XCD on one contiguous
+// range (runs of 16 tiles would leave some XCDs a run short), a larger one
+// deals the runs to the XCDs in turn.
+constexpr int kAutoChunkTiles = 1024;
+void resolve_tile_order(e3d_ctx* c) {
+    const int tiles = c->m.NRp / kWG;
+    c->tile_map = c->tile_pref != E3D_TILES_AUTO ? c->tile_pref
+                  : tiles < kAutoChunkTiles      ? E3D_TILES_XCD_CONTIGUOUS
+                                                 : E3D_TILES_XCD_CHUNKS;
+    c->m.tile_map = c->tile_map;
 }
 
 void free_device(e3d_ctx* c) {
@@ -623,7 +637,8 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     m.sSR = reinterpret_cast<const double2*>(d_sSR);
     m.sNA = reinterpret_cast<const double2*>(d_sNA);
     m.mate = d_mate; m.g_r = d_gr;
-    m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr; m.tile_map = c->tile_map;
+    m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr;
+    resolve_tile_order(c);
     DState& s = c->s;
     s.jac = nullptr;
     if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
@@ -1171,7 +1186,7 @@ int e3d_batch_history(e3d_ctx* c, int i, e3d_monitor* rows, int max_rows) {
     return n;
 }
 
-static_assert(E3D_TILES_LINEAR == 0 && E3D_TILES_XCD_CONTIGUOUS == 1 && E3D_TILES_XCD_CHUNKS == 2,
+static_assert(E3D_TILES_AUTO == -1 && E3D_TILES_LINEAR == 0 && E3D_TILES_XCD_CONTIGUOUS == 1 && E3D_TILES_XCD_CHUNKS == 2,
               "tile orders as tile_of numbers them");
 static_assert(E3D_ORDER_REFERENCE == ORDER_REFERENCE && E3D_ORDER_RCM == ORDER_RCM &&
                   E3D_ORDER_MORTON == ORDER_MORTON,
@@ -1186,9 +1201,10 @@ int e3d_set_ordering(e3d_ctx* c, int mode) {
 
 int e3d_set_tile_order(e3d_ctx* c, int order) {
     if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
-    if (order < E3D_TILES_LINEAR || order > E3D_TILES_XCD_CHUNKS)
-        return set_error(c, E3D_EINVAL, "e3d_set_tile_order: order must be 0, 1 or 2");
-    c->tile_map = c->m.tile_map = order;
+    if (order < E3D_TILES_AUTO || order > E3D_TILES_XCD_CHUNKS)
+        return set_error(c, E3D_EINVAL, "e3d_set_tile_order: order must be -1, 0, 1 or 2");
+    c->tile_pref = order;
+    resolve_tile_order(c);
     drop_graph(c);
     return E3D_OK;
 }

@@ -258,8 +258,11 @@ int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
  * reference, which runs one cell loop on one core). Workgroups are dealt to
  * the GPU's eight XCDs in turn; E3D_TILES_LINEAR maps workgroup i to tile i,
  * E3D_TILES_XCD_CONTIGUOUS gives each XCD one contiguous range of tiles,
- * E3D_TILES_XCD_CHUNKS (default) deals runs of 16 consecutive tiles to the
- * XCDs in turn. Addresses only: results are the same. */
+ * E3D_TILES_XCD_CHUNKS deals runs of 16 consecutive tiles to the XCDs in
+ * turn, E3D_TILES_AUTO (default) takes XCD_CONTIGUOUS for meshes of fewer
+ * than 1024 tiles of 256 cells and XCD_CHUNKS for larger ones. Addresses
+ * only: results are the same. */
+#define E3D_TILES_AUTO (-1)
 #define E3D_TILES_LINEAR 0
 #define E3D_TILES_XCD_CONTIGUOUS 1
 #define E3D_TILES_XCD_CHUNKS 2

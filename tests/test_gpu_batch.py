@@ -125,8 +125,10 @@ def test_batch_tile_order_does_not_change_results():
         for blk in range(len(a.sizes)):
             assert (a.get(blk, E.ARR_W).view(np.int64) == other.get(blk, E.ARR_W).view(np.int64)).all()
         assert a.batch_history(1, 20) == other.batch_history(1, 20)
-    with pytest.raises(E.E3DError):
-        a.set_tile_order(3)
+    for bad in (3, -2):
+        with pytest.raises(E.E3DError):
+            a.set_tile_order(bad)
+    a.set_tile_order(E.TILES_AUTO)
     for x in (a, b, c):
         x.close()
 
