@@ -189,6 +189,25 @@ def test_fused_medium_mesh_matches_oracle(scheme):
     ctx.close()
 
 
+@pytest.mark.parametrize("tiles", ["linear", "xcd_contiguous", "xcd_chunks"])
+def test_fused_tile_orders_match_oracle(tiles):
+    """Every workgroup-to-tile order gives the oracle's values: 48x16x16 hexes
+    (73,728 tets, 288 tiles) in 8 blocks, so the runs of 16 tiles do not divide
+    the tiles evenly over the eight XCDs."""
+    meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=3)
+    mesh = E.Mesh.bump(48, 16, 16, 0.1, 4, (2, 2, 2))
+    ctx = gpu_context(mesh, product_params(meta))
+    ctx.set_tile_order({"linear": E.TILES_LINEAR, "xcd_contiguous": E.TILES_XCD_CONTIGUOUS,
+                        "xcd_chunks": E.TILES_XCD_CHUNKS}[tiles])
+    hist = ctx.iterate(meta["iters"])
+    S = O.Solver([mesh.block(b) for b in range(mesh.nblocks)], oracle_params(meta))
+    S.init_freestream()
+    rh = S.iterate(meta["iters"], threads=8)
+    assert_states_equal(ctx, S)
+    assert_monitors_close(hist, rh)
+    ctx.close()
+
+
 def test_fused_is_deterministic_and_resumable():
     """Two runs give identical bits; 3 + 3 iterations equal 6 in one call."""
     meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=50)
