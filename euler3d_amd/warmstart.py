@@ -204,12 +204,16 @@ class Surrogate:
     further one fits what the ones before it leave (rescaled to unit size), so
     the smooth part of the error, which the solver removes slowly, shrinks."""
 
-    def __init__(self, X, T, stages=1, steps=30000, batch=8192, lr=2e-3, width=256, depth=5, log=sys.stderr):
+    def __init__(self, X, T, stages=1, steps=30000, batch=8192, lr=2e-3, width=256, depth=5, log=sys.stderr,
+                 var_weights=None):
+        """var_weights: relative loss weight of (rho, u, v, w, p), 1 each by default; a variable's
+        targets are divided by its weight, so its squared error counts weight^2 times."""
         self.nets, self.scales, self.loss = [], [], None
         t1 = time.perf_counter()
+        wv = np.ones(5, np.float32) if var_weights is None else np.asarray(var_weights, np.float32)
         R = T
         for stage in range(stages):
-            sc = np.ones(5, np.float32) if stage == 0 else R.std(axis=0).astype(np.float32) + 1e-12
+            sc = (np.ones(5, np.float32) if stage == 0 else R.std(axis=0).astype(np.float32) + 1e-12) / wv
             net = MLP(N_FEATURES, 5, width, depth, seed=1 + 10 * stage)
             net.set_data(X, R / sc)
             ls = net.train(steps, batch=batch, lr=lr, seed=2 + 10 * stage)
@@ -323,7 +327,7 @@ class ErrorMemory:
 
 def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000, criterion=-8.0, steps=3000,
         batch=8192, lr=2e-3, width=256, depth=5, deeper=3.0, train_criterion=None, boost=0, memory=False,
-        log=sys.stderr):
+        var_weights=None, log=sys.stderr):
     scheme = scheme_params(order)
     train_c = conditions(range(n_train))
     held_c = conditions(range(5000, 5000 + n_heldout))
@@ -333,7 +337,7 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
     tcrit = criterion if train_criterion is None else train_criterion
     X, T, res, t_gen, kept = training_set(train_c, nx, ny, nz, scheme, max_iter, tcrit, log)
     # 2. training on the GPU
-    sur = Surrogate(X, T, 1 + boost, steps, batch, lr, width, depth, log)
+    sur = Surrogate(X, T, 1 + boost, steps, batch, lr, width, depth, log, var_weights)
     loss, t_train, ms = sur.loss, sur.train_seconds, sur.step_ms
     flops = sur.tflops * ms * 1e9
     train_max_err, train_mean_err = sur.train_max_err, sur.train_mean_err
@@ -410,6 +414,7 @@ def run(n_train=128, n_heldout=16, nx=48, ny=16, nz=16, order=1, max_iter=30000,
                      cd_deep_rel_diff=abs(dw["cd"] - dc["cd"]) / max(abs(dc["cd"]), 1e-300))
     both = [r for r in rows if r["cold_converged"] and r["warm_converged"]]
     summary = dict(n_train=n_train, n_heldout=n_heldout, mesh=[nx, ny, nz], order=order, criterion=criterion,
+                   var_weights=None if var_weights is None else [float(x) for x in var_weights],
                    train_criterion=tcrit, width=width, boost=boost, by_stages=by_stages, train_max_err=train_max_err,
                    train_mean_err=train_mean_err,
                    memory=(dict(method=mem.method, leave_one_out=mem.loo, fields=int(mem.R.shape[0]),
@@ -570,6 +575,8 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--boost", type=int, default=0,
                     help="further networks, each fitted to what the ones before leave")
+    ap.add_argument("--var-weights", type=lambda t: [float(x) for x in t.split(",")], default=None,
+                    help="loss weights of rho,u,v,w,p (default 1,1,1,1,1)")
     ap.add_argument("--memory", action="store_true",
                     help="add the recalled leftovers of the nearest training fields to the networks' field")
     ap.add_argument("--anatomy", action="store_true",
@@ -594,7 +601,8 @@ def main(argv=None):
                 json.dump(s, f, indent=1)
         return s
     s = run(a.train, a.heldout, a.nx, a.nx // 3, a.nx // 3, a.order, a.max_iter, a.criterion, a.steps, a.batch, a.lr,
-            width=a.width, deeper=a.deeper, train_criterion=a.train_criterion, boost=a.boost, memory=a.memory)
+            width=a.width, deeper=a.deeper, train_criterion=a.train_criterion, boost=a.boost, memory=a.memory,
+            var_weights=a.var_weights)
     print(f"[warmstart] held-out converged in both: {s['converged_both']}/{a.heldout}; cold/warm iterations "
           f"{s['iterations_ratio']:.2f}x (each run from its own iteration-0 residual: "
           f"{s['iterations_ratio_own_r0']:.2f}x), time {s['time_ratio']:.2f}x; Cl/Cd rel diff at the criterion "
