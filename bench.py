@@ -349,7 +349,9 @@ def main():
                                f"{mesh.nblocks} block(s) on one device, {sc['label']}, 5-stage RK, CFL 1.5",
                    "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages,
                    "cell_numbering": "scrambled" if a.brick < 0 else f"brick {a.brick}",
-                   "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate"},
+                   "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate",
+                   "tile_order": "automatic by mesh size (XCD-contiguous below 1024 tiles of 256 cells, "
+                                 "runs of 16 tiles dealt to the XCDs above)"},
         "model_bytes_per_cell_stage": model_bytes,
         "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / model_bytes / 1e6),
         "roofline": rooflines[dominant] if dominant else None,
