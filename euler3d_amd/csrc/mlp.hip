@@ -322,6 +322,16 @@ void gemm(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda,
                        K / splits, A, lda, B, ldb, e);
 }
 
+// Wider column tiles (4 MFMA tiles per wave) where N allows: each A operand
+// loaded feeds twice as many MFMAs. The accumulation order of every output
+// element is the same, so results are unchanged.
+template <int EPI>
+void gemm_n(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda, const bf16_t* B, int ldb,
+            const EpiArgs& e) {
+    if (N % 64 == 0) gemm<EPI, 4>(m, M, N, K, splits, A, lda, B, ldb, e);
+    else gemm<EPI, 2>(m, M, N, K, splits, A, lda, B, ldb, e);
+}
+
 // Forward pass of B samples already in X0 (bf16 [B x 32]); output fp32 in m->out [B x 32].
 void forward(e3d_mlp* m, int B) {
     const int w = m->width;
@@ -329,10 +339,10 @@ void forward(e3d_mlp* m, int B) {
         const bf16_t* in = l == 0 ? m->X0 : m->Y[l - 1];
         const int K = l == 0 ? kPad : w;
         EpiArgs e{m->b[l], m->Z[l], m->Y[l], m->Yt[l], nullptr};
-        gemm<EPI_FWD_SILU, 2>(m, B, w, K, 1, in, K, m->Wt[l], K, e);
+        gemm_n<EPI_FWD_SILU>(m, B, w, K, 1, in, K, m->Wt[l], K, e);
     }
     EpiArgs e{m->b[m->depth], nullptr, nullptr, nullptr, m->out};
-    gemm<EPI_FWD_OUT, 2>(m, B, kPad, w, 1, m->Y[m->depth - 1], w, m->Wt[m->depth], w, e);
+    gemm_n<EPI_FWD_OUT>(m, B, kPad, w, 1, m->Y[m->depth - 1], w, m->Wt[m->depth], w, e);
 }
 
 // Gradients of the mean squared error over B samples (targets in m->Tb) into gW / gb.
@@ -343,25 +353,25 @@ void backward(e3d_mlp* m, int B) {
     // output layer: dW [w x 32] (in x out) = Y_{L-1}^T dO; db = column sums of dO
     {
         EpiArgs e{nullptr, nullptr, nullptr, nullptr, m->part_w};
-        gemm<EPI_STORE, 2>(m, w, kPad, B, kSplits, m->Yt[L - 1], B, m->dOt, B, e);
+        gemm_n<EPI_STORE>(m, w, kPad, B, kSplits, m->Yt[L - 1], B, m->dOt, B, e);
         hipLaunchKernelGGL(k_sum_splits, dim3(cdiv((long)kPad * w, kThreads)), dim3(kThreads), 0, m->stream, m->part_w,
                            kSplits, kPad, w, 1, m->gW[L]);
         hipLaunchKernelGGL(k_row_sum, dim3(kPad), dim3(kThreads), 0, m->stream, m->dOt, B, m->gb[L]);
         EpiArgs d{nullptr, m->Z[L - 1], m->dZ[L - 1], m->dZt[L - 1], nullptr};
-        gemm<EPI_BWD_SILU, 2>(m, B, w, kPad, 1, m->dO, kPad, m->Wm[L], kPad, d);
+        gemm_n<EPI_BWD_SILU>(m, B, w, kPad, 1, m->dO, kPad, m->Wm[L], kPad, d);
     }
     for (int l = L - 1; l >= 0; --l) {
         const int K = l == 0 ? kPad : w;
         const bf16_t* in_t = l == 0 ? m->X0t : m->Yt[l - 1];
         // dW [w x K] (out x in) = dZ_l^T X_{l-1}
         EpiArgs e{nullptr, nullptr, nullptr, nullptr, m->part_w};
-        gemm<EPI_STORE, 2>(m, w, K, B, kSplits, m->dZt[l], B, in_t, B, e);
+        gemm_n<EPI_STORE>(m, w, K, B, kSplits, m->dZt[l], B, in_t, B, e);
         hipLaunchKernelGGL(k_sum_splits, dim3(cdiv((long)w * K, kThreads)), dim3(kThreads), 0, m->stream, m->part_w,
                            kSplits, w, K, 0, m->gW[l]);
         hipLaunchKernelGGL(k_row_sum, dim3(w), dim3(kThreads), 0, m->stream, m->dZt[l], B, m->gb[l]);
         if (l > 0) {
             EpiArgs d{nullptr, m->Z[l - 1], m->dZ[l - 1], m->dZt[l - 1], nullptr};
-            gemm<EPI_BWD_SILU, 2>(m, B, w, w, 1, m->dZ[l], w, m->Wm[l], w, d);
+            gemm_n<EPI_BWD_SILU>(m, B, w, w, 1, m->dZ[l], w, m->Wm[l], w, d);
         }
     }
 }
