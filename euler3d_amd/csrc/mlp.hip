@@ -61,6 +61,38 @@ struct EpiArgs {
     float* F;           // [M x N] fp32 result (output layer; weight-gradient partials, one [M x N] per split)
 };
 
+// Epilogue of one 16 x 16 MFMA tile: the lane holds C[mb + q][n], q = 0..3.
+template <int EPI>
+__device__ __forceinline__ void epilogue(const EpiArgs& e, int M, int N, int mb, int n, const floatx4& acc) {
+    bf16_t t[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int m = mb + q;
+        const size_t o = (size_t)m * N + n;
+        float v = acc[q];
+        if (EPI == EPI_FWD_SILU) {
+            v += e.bias[n];
+            e.Z[o] = v;
+            const float y = v * sigmoid(v);
+            t[q] = to_bf16(y);
+            e.Y[o] = t[q];
+        } else if (EPI == EPI_FWD_OUT) {
+            e.F[o] = v + e.bias[n];
+        } else if (EPI == EPI_BWD_SILU) {
+            const float z = e.Z[o], s = sigmoid(z);
+            const float d = v * (s + z * s * (1.0f - s));
+            t[q] = to_bf16(d);
+            e.Y[o] = t[q];
+        } else {
+            e.F[(size_t)blockIdx.z * M * N + o] = v;
+        }
+    }
+    if (EPI == EPI_FWD_SILU || EPI == EPI_BWD_SILU) {
+        const uint64_t packed = (uint64_t)t[0] | ((uint64_t)t[1] << 16) | ((uint64_t)t[2] << 32) | ((uint64_t)t[3] << 48);
+        *reinterpret_cast<uint64_t*>(e.Yt + (size_t)n * M + mb) = packed;
+    }
+}
+
 // C[M x N] = A[M x K] * B[N x K]^T over K range [k0, k1) of split blockIdx.z.
 // Workgroup: 4 waves, rows [128 x, +128), columns [16*TN y, +16*TN); wave w
 // computes rows +32w .. +32w+31 as two 16-row MFMA tiles.
@@ -94,38 +126,68 @@ __global__ void __launch_bounds__(kThreads) k_gemm(int M, int N, int ksplit, con
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int n = n0 + 16 * j + r;
-            const int mb = m0 + 16 * i + 4 * g;
-            bf16_t t[4];
+        for (int j = 0; j < TN; ++j) epilogue<EPI>(e, M, N, m0 + 16 * i + 4 * g, n0 + 16 * j + r, acc[i][j]);
+}
+
+// The same product with the workgroup's operand tiles staged in LDS: a
+// 128 x 128 output tile per workgroup, 64 x 64 (4 x 4 MFMA tiles) per wave,
+// so each 16-byte operand fetched from memory feeds 16 MFMAs of the
+// workgroup instead of 2-4. The next k-step's tiles are loaded into registers
+// while this step's MFMAs run. Every output element accumulates over k in
+// the same order as k_gemm, so the two give the same bits. M and N must be
+// multiples of 128.
+template <int EPI>
+__global__ void __launch_bounds__(kThreads) k_gemm_lds(int M, int N, int ksplit, const bf16_t* __restrict__ A,
+                                                       int lda, const bf16_t* __restrict__ B, int ldb, EpiArgs e) {
+    constexpr int T = 128, KS = 32, LD = KS + 8;  // tile, k-step, LDS row stride in bf16 (80 bytes)
+    __shared__ bf16_t As[T * LD], Bs[T * LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int mt = blockIdx.x * T, nt = blockIdx.y * T;
+    const int k0 = blockIdx.z * ksplit, k1 = k0 + ksplit;
+    const int r = lane & 15, g = lane >> 4;
+    // this thread's two 16-byte chunks of each operand tile: rows tid / 4 and
+    // 64 + tid / 4, columns 8 (tid % 4) .. + 7 of the k-step
+    const int row0 = tid >> 2, row1 = row0 + 64, part = (tid & 3) * 8;
+    floatx4 acc[4][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int m = mb + q;
-                const size_t o = (size_t)m * N + n;
-                float v = acc[i][j][q];
-                if (EPI == EPI_FWD_SILU) {
-                    v += e.bias[n];
-                    e.Z[o] = v;
-                    const float y = v * sigmoid(v);
-                    t[q] = to_bf16(y);
-                    e.Y[o] = t[q];
-                } else if (EPI == EPI_FWD_OUT) {
-                    e.F[o] = v + e.bias[n];
-                } else if (EPI == EPI_BWD_SILU) {
-                    const float z = e.Z[o], s = sigmoid(z);
-                    const float d = v * (s + z * s * (1.0f - s));
-                    t[q] = to_bf16(d);
-                    e.Y[o] = t[q];
-                } else {
-                    e.F[(size_t)blockIdx.z * M * N + o] = v;
-                }
-            }
-            if (EPI == EPI_FWD_SILU || EPI == EPI_BWD_SILU) {
-                uint64_t packed = (uint64_t)t[0] | ((uint64_t)t[1] << 16) | ((uint64_t)t[2] << 32) |
-                                  ((uint64_t)t[3] << 48);
-                *reinterpret_cast<uint64_t*>(e.Yt + (size_t)n * M + mb) = packed;
-            }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bf16_t* a0 = A + (size_t)(mt + row0) * lda + part;
+    const bf16_t* a1 = A + (size_t)(mt + row1) * lda + part;
+    const bf16_t* b0 = B + (size_t)(nt + row0) * ldb + part;
+    const bf16_t* b1 = B + (size_t)(nt + row1) * ldb + part;
+    bf16x8 ga0 = *reinterpret_cast<const bf16x8*>(a0 + k0), ga1 = *reinterpret_cast<const bf16x8*>(a1 + k0);
+    bf16x8 gb0 = *reinterpret_cast<const bf16x8*>(b0 + k0), gb1 = *reinterpret_cast<const bf16x8*>(b1 + k0);
+    for (int k = k0; k < k1; k += KS) {
+        __syncthreads();  // the previous step's LDS reads are done
+        *reinterpret_cast<bf16x8*>(As + row0 * LD + part) = ga0;
+        *reinterpret_cast<bf16x8*>(As + row1 * LD + part) = ga1;
+        *reinterpret_cast<bf16x8*>(Bs + row0 * LD + part) = gb0;
+        *reinterpret_cast<bf16x8*>(Bs + row1 * LD + part) = gb1;
+        __syncthreads();
+        if (k + KS < k1) {  // uniform: prefetch the next k-step
+            ga0 = *reinterpret_cast<const bf16x8*>(a0 + k + KS);
+            ga1 = *reinterpret_cast<const bf16x8*>(a1 + k + KS);
+            gb0 = *reinterpret_cast<const bf16x8*>(b0 + k + KS);
+            gb1 = *reinterpret_cast<const bf16x8*>(b1 + k + KS);
         }
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(As + (wm * 64 + 16 * i + r) * LD + 8 * g);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn * 64 + 16 * j + r) * LD + 8 * g);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            epilogue<EPI>(e, M, N, mt + wm * 64 + 16 * i + 4 * g, nt + wn * 64 + 16 * j + r, acc[i][j]);
 }
 
 // Mini-batch assembly: features (padded with zeros to 32) as bf16, row-major and
@@ -328,7 +390,10 @@ void gemm(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda,
 template <int EPI>
 void gemm_n(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda, const bf16_t* B, int ldb,
             const EpiArgs& e) {
-    if (N % 64 == 0) gemm<EPI, 4>(m, M, N, K, splits, A, lda, B, ldb, e);
+    if (M % 128 == 0 && N % 128 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
+                           K / splits, A, lda, B, ldb, e);
+    else if (N % 64 == 0) gemm<EPI, 4>(m, M, N, K, splits, A, lda, B, ldb, e);
     else gemm<EPI, 2>(m, M, N, K, splits, A, lda, B, ldb, e);
 }
 
