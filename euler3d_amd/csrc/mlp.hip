@@ -240,7 +240,17 @@ __global__ void k_row_sum(const bf16_t* Gt, int B, float* out) {
     __shared__ float red[kThreads];
     const int n = blockIdx.x;
     float s = 0.0f;
-    for (int b = threadIdx.x; b < B; b += kThreads) s += from_bf16(Gt[(size_t)n * B + b]);
+    // eight loads in flight, then added in the same order as one at a time
+    const bf16_t* row = Gt + (size_t)n * B;
+    int b = threadIdx.x;
+    for (; b + 7 * kThreads < B; b += 8 * kThreads) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = from_bf16(row[b + u * kThreads]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; b < B; b += kThreads) s += from_bf16(row[b]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int o = kThreads / 2; o > 0; o >>= 1) {
@@ -258,7 +268,16 @@ __global__ void k_sum_splits(const float* part, int splits, int n_out, int n_in,
     const int o = x / n_in, i = x % n_in;
     const size_t src = transposed ? (size_t)i * n_out + o : (size_t)x;
     float s = 0.0f;
-    for (int z = 0; z < splits; ++z) s += part[(size_t)z * n_out * n_in + src];
+    const size_t stride = (size_t)n_out * n_in;
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {  // eight loads in flight, added in order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(z + u) * stride + src];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; z < splits; ++z) s += part[(size_t)z * stride + src];
     grad[x] = s;
 }
 
