@@ -192,17 +192,24 @@ __global__ void __launch_bounds__(kThreads) k_gemm_lds(int M, int N, int ksplit,
 
 // Mini-batch assembly: features (padded with zeros to 32) as bf16, row-major and
 // transposed, and the fp32 targets, for samples idx[0..B).
+// A workgroup takes 64 samples; each of its four waves a quarter of the 32
+// padded features of all 64 (so a wave's transposed stores of one feature are
+// 64 consecutive samples).
+constexpr int kGatherSamples = kThreads / 4;
 __global__ void k_gather(const float* X, const float* T, const int* idx, int n_in, int n_out, int B, bf16_t* X0,
                          bf16_t* X0t, float* Tb) {
-    const int b = blockIdx.x * kThreads + threadIdx.x;
+    const int b = blockIdx.x * kGatherSamples + (threadIdx.x & (kGatherSamples - 1));
+    const int quarter = threadIdx.x / kGatherSamples;
     if (b >= B) return;
     const size_t i = (size_t)idx[b];
-    for (int k = 0; k < kPad; ++k) {
+#pragma unroll
+    for (int u = 0; u < kPad / 4; ++u) {
+        const int k = quarter * (kPad / 4) + u;
         const bf16_t v = to_bf16(k < n_in ? X[i * n_in + k] : 0.0f);
         X0[(size_t)b * kPad + k] = v;
         X0t[(size_t)k * B + b] = v;
     }
-    for (int j = 0; j < n_out; ++j) Tb[(size_t)b * n_out + j] = T[i * n_out + j];
+    for (int j = quarter; j < n_out; j += 4) Tb[(size_t)b * n_out + j] = T[i * n_out + j];
 }
 
 // Mean squared error over B samples and n_out outputs: per-workgroup partial sums,
@@ -610,7 +617,7 @@ int e3d_mlp_gradients(e3d_mlp* m, const int* idx, int B, float* loss, float* gra
     for (int b = 0; b < B; ++b)
         if (idx[b] < 0 || idx[b] >= m->n_data) return fail(m, E3D_EINVAL, "e3d_mlp_gradients: sample index out of range");
     MLP_TRY(m, hipMemcpyAsync(m->idx, idx, sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
                        m->n_out, B, m->X0, m->X0t, m->Tb);
     forward(m, B);
     backward(m, B);
@@ -658,7 +665,7 @@ int e3d_mlp_train(e3d_mlp* m, int steps, int B, float lr, int cosine, unsigned l
             batch[b] = order[pos++];
         }
         MLP_TRY(m, hipMemcpyAsync(m->idx, batch.data(), sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
-        hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
+        hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
                            m->n_out, B, m->X0, m->X0t, m->Tb);
         forward(m, B);
         backward(m, B);
@@ -706,7 +713,7 @@ int e3d_mlp_predict(e3d_mlp* m, const float* x, long n, float* y) {
             (void)hipFree(dx);
             return fail(m, E3D_EHIP, std::string("e3d_mlp_predict: ") + hipGetErrorString(e));
         }
-        hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, dx, m->T ? m->T : dx, m->idx,
+        hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, dx, m->T ? m->T : dx, m->idx,
                            m->n_in, 0, B, m->X0, m->X0t, m->Tb);
         forward(m, B);
         e = hipMemcpyAsync(out.data(), m->out, sizeof(float) * (size_t)B * kPad, hipMemcpyDeviceToHost, m->stream);
@@ -733,7 +740,7 @@ int e3d_mlp_time_step(e3d_mlp* m, int B, int reps, double* ms, double* flops) {
     std::vector<int> idx(B);
     for (int b = 0; b < B; ++b) idx[b] = (int)(b % m->n_data);
     MLP_TRY(m, hipMemcpyAsync(m->idx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
                        m->n_out, B, m->X0, m->X0t, m->Tb);
     hipEvent_t a, b;
     MLP_TRY(m, hipEventCreate(&a));
