@@ -437,10 +437,12 @@ void forward(e3d_mlp* m, int B) {
 }
 
 // Gradients of the mean squared error over B samples (targets in m->Tb) into gW / gb.
-void backward(e3d_mlp* m, int B) {
+// part_loss: where the loss kernel leaves its per-workgroup partial sums
+// (m->part_loss by default).
+void backward(e3d_mlp* m, int B, float* part_loss = nullptr) {
     const int w = m->width, L = m->depth;
     hipLaunchKernelGGL(k_loss, dim3(cdiv(B, kThreads)), dim3(kThreads), 0, m->stream, m->out, m->Tb, m->n_out, B, m->dO,
-                       m->dOt, m->part_loss);
+                       m->dOt, part_loss ? part_loss : m->part_loss);
     // output layer: dW [w x 32] (in x out) = Y_{L-1}^T dO; db = column sums of dO
     {
         EpiArgs e{nullptr, nullptr, nullptr, nullptr, m->part_w};
@@ -649,46 +651,70 @@ int e3d_mlp_train(e3d_mlp* m, int steps, int B, float lr, int cosine, unsigned l
     if (rc) return rc;
     MLP_TRY(m, hipSetDevice(m->device));
     if ((rc = ensure_work(m, B))) return rc;
+    // Steps run in chunks: the sample indices of a whole chunk go up in one
+    // copy and the loss partials of each step stay on the device until the
+    // chunk ends, so the host waits once per chunk, not once per step. The
+    // indices, their order and every kernel are those of one step at a time.
+    constexpr int kChunk = 32;
+    const int nparts = cdiv(B, kThreads);
+    int* d_idx = nullptr;
+    float* d_part = nullptr;
+    MLP_TRY(m, hipMalloc(&d_idx, sizeof(int) * (size_t)kChunk * B));
+    hipError_t ea = hipMalloc(&d_part, sizeof(float) * (size_t)kChunk * nparts);
+    if (ea != hipSuccess) {
+        (void)hipFree(d_idx);
+        return fail(m, E3D_EHIP, std::string("e3d_mlp_train: ") + hipGetErrorString(ea));
+    }
     std::mt19937_64 rng(seed);
     std::vector<int> order(m->n_data);
     for (long i = 0; i < m->n_data; ++i) order[i] = (int)i;
     long pos = m->n_data;
-    std::vector<int> batch(B);
-    std::vector<float> part(cdiv(B, kThreads));
+    std::vector<int> batch((size_t)kChunk * B);
+    std::vector<float> part((size_t)kChunk * nparts);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int s = 0; s < steps; ++s) {
-        for (int b = 0; b < B; ++b) {
+    hipError_t e = hipSuccess;
+    for (int s0 = 0; s0 < steps && e == hipSuccess; s0 += kChunk) {
+        const int n = std::min(kChunk, steps - s0);
+        for (size_t b = 0; b < (size_t)n * B; ++b) {
             if (pos >= m->n_data) {
                 std::shuffle(order.begin(), order.end(), rng);
                 pos = 0;
             }
             batch[b] = order[pos++];
         }
-        MLP_TRY(m, hipMemcpyAsync(m->idx, batch.data(), sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
-        hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, m->X, m->T, m->idx, m->n_in,
-                           m->n_out, B, m->X0, m->X0t, m->Tb);
-        forward(m, B);
-        backward(m, B);
-        ++m->step;
-        const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
-        const float rate = cosine ? lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * s / std::max(1, steps))))
-                                  : lr;
-        for (int l = 0; l < m->nl; ++l)
-            hipLaunchKernelGGL(k_adam, dim3(cdiv((long)m->dout[l] * m->din[l], kThreads)), dim3(kThreads), 0, m->stream,
-                               m->W[l], m->mW[l], m->vW[l], m->gW[l], m->b[l], m->mb[l], m->vb[l], m->gb[l], m->dout[l],
-                               m->din[l], m->rout[l], m->rin[l], rate, b1, b2, eps, c1, c2, m->Wt[l], m->Wm[l]);
-        MLP_TRY(m, hipGetLastError());
-        if (loss) {
-            MLP_TRY(m, hipMemcpyAsync(part.data(), m->part_loss, sizeof(float) * part.size(), hipMemcpyDeviceToHost,
-                                      m->stream));
-            MLP_TRY(m, hipStreamSynchronize(m->stream));  // the index upload above reuses `batch`
-            double t = 0.0;
-            for (float p : part) t += p;
-            loss[s] = (float)(t / ((double)B * m->n_out));
-        } else {
-            MLP_TRY(m, hipStreamSynchronize(m->stream));
+        e = hipMemcpyAsync(d_idx, batch.data(), sizeof(int) * (size_t)n * B, hipMemcpyHostToDevice, m->stream);
+        for (int j = 0; j < n && e == hipSuccess; ++j) {
+            const int s = s0 + j;
+            hipLaunchKernelGGL(k_gather, dim3(cdiv(B, kGatherSamples)), dim3(kThreads), 0, m->stream, m->X, m->T,
+                               d_idx + (size_t)j * B, m->n_in, m->n_out, B, m->X0, m->X0t, m->Tb);
+            forward(m, B);
+            backward(m, B, d_part + (size_t)j * nparts);
+            ++m->step;
+            const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
+            const float rate =
+                cosine ? lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * s / std::max(1, steps)))) : lr;
+            for (int l = 0; l < m->nl; ++l)
+                hipLaunchKernelGGL(k_adam, dim3(cdiv((long)m->dout[l] * m->din[l], kThreads)), dim3(kThreads), 0,
+                                   m->stream, m->W[l], m->mW[l], m->vW[l], m->gW[l], m->b[l], m->mb[l], m->vb[l],
+                                   m->gb[l], m->dout[l], m->din[l], m->rout[l], m->rin[l], rate, b1, b2, eps, c1, c2,
+                                   m->Wt[l], m->Wm[l]);
+            e = hipGetLastError();
         }
+        if (e == hipSuccess && loss)
+            e = hipMemcpyAsync(part.data(), d_part, sizeof(float) * (size_t)n * nparts, hipMemcpyDeviceToHost,
+                               m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);  // before `batch` and `part` are reused
+        if (e == hipSuccess && loss)
+            for (int j = 0; j < n; ++j) {
+                double t = 0.0;
+                for (int q = 0; q < nparts; ++q) t += part[(size_t)j * nparts + q];
+                loss[s0 + j] = (float)(t / ((double)B * m->n_out));
+            }
     }
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_part);
+    if (e != hipSuccess) return fail(m, E3D_EHIP, std::string("e3d_mlp_train: ") + hipGetErrorString(e));
     return E3D_OK;
 }
 
