@@ -142,6 +142,7 @@ def lib():
             "e3d_mlp_gradients": (C.c_int, [C.c_void_p, _ip, C.c_int, _fp, _fp]),
             "e3d_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_ulonglong, _fp]),
             "e3d_mlp_predict": (C.c_int, [C.c_void_p, _fp, C.c_long, _fp]),
+            "e3d_mlp_predict_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
             "e3d_mlp_time_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
         }
         for name in ("e3d_save_w0", "e3d_spectral_radius", "e3d_timestep", "e3d_interpolation",
@@ -426,6 +427,11 @@ class MLP:
         y = np.empty((len(x), self.n_out), np.float32)
         _check(lib().e3d_mlp_predict(self._h, x.ctypes.data_as(_fp), len(x), y.ctypes.data_as(_fp)))
         return y
+
+    def predict_noalloc(self, x_ptr, n, y_ptr):
+        """Outputs for n feature rows already in device memory (x_ptr [n x n_in],
+        y_ptr [n x n_out], float32, addresses as integers); allocates nothing."""
+        _check(lib().e3d_mlp_predict_noalloc(self._h, C.c_void_p(x_ptr), n, C.c_void_p(y_ptr)))
 
     def time_step(self, batch, reps=10):
         ms, fl = C.c_double(), C.c_double()

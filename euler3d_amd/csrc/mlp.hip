@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -51,7 +52,10 @@ __host__ __device__ inline float from_bf16(bf16_t h) {
 }
 __device__ inline float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
 
-enum Epi { EPI_FWD_SILU = 0, EPI_FWD_OUT = 1, EPI_BWD_SILU = 2, EPI_STORE = 3 };
+// EPI_FWD_INFER: the SiLU layer of EPI_FWD_SILU with the same arithmetic,
+// storing only the bf16 activation (inference keeps no pre-activations and
+// no transposed copy).
+enum Epi { EPI_FWD_SILU = 0, EPI_FWD_OUT = 1, EPI_BWD_SILU = 2, EPI_STORE = 3, EPI_FWD_INFER = 4 };
 
 struct EpiArgs {
     const float* bias;  // [N] (forward)
@@ -76,6 +80,10 @@ __device__ __forceinline__ void epilogue(const EpiArgs& e, int M, int N, int mb,
             const float y = v * sigmoid(v);
             t[q] = to_bf16(y);
             e.Y[o] = t[q];
+        } else if (EPI == EPI_FWD_INFER) {
+            v += e.bias[n];
+            const float y = v * sigmoid(v);
+            e.Y[o] = to_bf16(y);
         } else if (EPI == EPI_FWD_OUT) {
             e.F[o] = v + e.bias[n];
         } else if (EPI == EPI_BWD_SILU) {
@@ -324,9 +332,27 @@ __global__ void k_copy_weights(const float* W, int n_out, int n_in, bf16_t* Wt, 
     Wm[(size_t)i * n_out + o] = h;
 }
 
+// Inference input: rows [0, cnt) of x [cnt x n_in] as bf16 into X0 [R x 32],
+// zero features beyond n_in and zero rows beyond cnt (every entry written).
+__global__ void k_pack_rows(const float* x, int cnt, int R, int n_in, bf16_t* X0) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= R * kPad) return;
+    const int b = t / kPad, k = t % kPad;
+    X0[t] = to_bf16(b < cnt && k < n_in ? x[(size_t)b * n_in + k] : 0.0f);
+}
+
+// Inference output: the first n_out columns of out [cnt x 32] into y [cnt x n_out].
+__global__ void k_unpack_rows(const float* out, int cnt, int n_out, float* y) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= cnt * n_out) return;
+    const int b = t / n_out, j = t % n_out;
+    y[t] = out[(size_t)b * kPad + j];
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
+
 
 struct e3d_mlp {
     int device = 0;
@@ -473,6 +499,78 @@ int check_batch(e3d_mlp* m, long B) {
     if (B <= 0 || B % kRows != 0 || B % (32 * kSplits) != 0 || B > (1L << 24))
         return fail(m, E3D_EINVAL, "mini-batch size must be a positive multiple of 512 (and at most 2^24)");
     return E3D_OK;
+}
+
+}  // namespace
+
+// Scratch of e3d_mlp_predict_noalloc: the bf16 input rows, two bf16
+// activation buffers and the fp32 output rows of one chunk. One copy per GPU
+// for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_address below).
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current: looked up
+// once per device, then reused.
+int scratch_address(e3d_mlp* m, int device, void** p) {
+    if (device < 0 || device >= kMaxDevices) return fail(m, E3D_EINVAL, "e3d_mlp_predict_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q)
+            return fail(m, E3D_EHIP, std::string("e3d_mlp_predict_noalloc: hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_ptr[device] = q;
+    }
+    *p = g_scratch_ptr[device];
+    return E3D_OK;
+}
+
+// Rows per chunk of e3d_mlp_predict_noalloc for a network of this width: the
+// largest multiple of 512 whose buffers fit the scratch (0 if none does).
+long noalloc_rows(const e3d_mlp* m) {
+    const size_t per_row = sizeof(bf16_t) * kPad + 2 * sizeof(bf16_t) * (size_t)m->width + sizeof(float) * kPad;
+    return (long)(kScratchBytes / per_row) / 512 * 512;
+}
+
+// Accepts only memory of the network's device (a host pointer in a kernel
+// argument would fault the GPU).
+int check_device_ptr(e3d_mlp* m, const void* p, const char* what) {
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(m, E3D_EINVAL, std::string("e3d_mlp_predict_noalloc: ") + what + " is not device memory");
+    }
+    if ((a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != m->device)
+        return fail(m, E3D_EINVAL, std::string("e3d_mlp_predict_noalloc: ") + what +
+                                       " must be device memory of the network's GPU " + std::to_string(m->device));
+    return E3D_OK;
+}
+
+// Inference forward pass of B rows of X0 with the activations alternating
+// between Ya and Yb; fp32 outputs [B x 32] into out. The same GEMMs and
+// arithmetic as forward(), so the outputs have the same bits.
+void forward_infer(e3d_mlp* m, int B, const bf16_t* X0, bf16_t* Ya, bf16_t* Yb, float* out) {
+    const int w = m->width;
+    const bf16_t* in = X0;
+    bf16_t *cur = Ya, *nxt = Yb;
+    for (int l = 0; l < m->depth; ++l) {
+        const int K = l == 0 ? kPad : w;
+        EpiArgs e{m->b[l], nullptr, cur, nullptr, nullptr};
+        gemm_n<EPI_FWD_INFER>(m, B, w, K, 1, in, K, m->Wt[l], K, e);
+        in = cur;
+        std::swap(cur, nxt);
+    }
+    EpiArgs e{m->b[m->depth], nullptr, nullptr, nullptr, out};
+    gemm_n<EPI_FWD_OUT>(m, B, kPad, w, 1, in, w, m->Wt[m->depth], w, e);
 }
 
 }  // namespace
@@ -752,6 +850,43 @@ int e3d_mlp_predict(e3d_mlp* m, const float* x, long n, float* y) {
             for (int j = 0; j < m->n_out; ++j) y[(s + b) * m->n_out + j] = out[(size_t)b * kPad + j];
     }
     (void)hipFree(dx);
+    return E3D_OK;
+}
+
+// e3d_mlp_predict on device buffers without allocating: each chunk of rows
+// goes through the static scratch (packed inputs, activations, outputs).
+// Every scratch entry a chunk reads is written earlier in the same chunk, so
+// nothing needs clearing between calls; calls that overlap on one GPU share
+// the scratch and race.
+int e3d_mlp_predict_noalloc(e3d_mlp* m, const float* x, long n, float* y) {
+    if (!m || !x || !y || n < 0) return e3d::set_error(nullptr, E3D_EINVAL, "e3d_mlp_predict_noalloc: bad argument");
+    MLP_TRY(m, hipSetDevice(m->device));
+    const long R = noalloc_rows(m);
+    if (R < 512)
+        return fail(m, E3D_EINVAL, "e3d_mlp_predict_noalloc: width " + std::to_string(m->width) +
+                                       " does not fit the " + std::to_string(kScratchBytes >> 20) +
+                                       " MiB scratch (512 rows need " +
+                                       std::to_string(512 * (2 * kPad + 4 * (size_t)m->width + 4 * kPad)) + " bytes)");
+    int rc;
+    if ((rc = check_device_ptr(m, x, "x")) || (rc = check_device_ptr(m, y, "y"))) return rc;
+    void* base = nullptr;
+    if ((rc = scratch_address(m, m->device, &base))) return rc;
+    bf16_t* X0 = static_cast<bf16_t*>(base);
+    bf16_t* Ya = X0 + (size_t)R * kPad;
+    bf16_t* Yb = Ya + (size_t)R * m->width;
+    float* out = reinterpret_cast<float*>(Yb + (size_t)R * m->width);
+    for (long s = 0; s < n; s += R) {
+        const int cnt = (int)std::min<long>(R, n - s);
+        const int B = (cnt + 511) / 512 * 512;
+        hipLaunchKernelGGL(k_pack_rows, dim3(cdiv((long)B * kPad, kThreads)), dim3(kThreads), 0, m->stream,
+                           x + (size_t)s * m->n_in, cnt, B, m->n_in, X0);
+        forward_infer(m, B, X0, Ya, Yb, out);
+        hipLaunchKernelGGL(k_unpack_rows, dim3(cdiv((long)cnt * m->n_out, kThreads)), dim3(kThreads), 0, m->stream, out,
+                           cnt, m->n_out, y + (size_t)s * m->n_out);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) return fail(m, E3D_EHIP, std::string("e3d_mlp_predict_noalloc: ") + hipGetErrorString(e));
     return E3D_OK;
 }
 

@@ -319,6 +319,18 @@ int e3d_mlp_gradients(e3d_mlp* mlp, const int* idx, int batch, float* loss, floa
 int e3d_mlp_train(e3d_mlp* mlp, int steps, int batch, float lr, int cosine, unsigned long long seed, float* loss);
 /* Outputs y [n x n_out] for features x [n x n_in] (host arrays). */
 int e3d_mlp_predict(e3d_mlp* mlp, const float* x, long n, float* y);
+/* e3d_mlp_predict for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); it
+ * has no counterpart in the reference. x [n x n_in] and y [n x n_out] are
+ * device memory of the network's GPU, filled and read by the caller. The
+ * library allocates nothing in this call: rows go through a static per-GPU
+ * scratch of 8 MiB in chunks of the largest multiple of 512 rows that fits
+ * (3,584 rows at width 512), so any n is accepted; widths whose 512 rows do
+ * not fit (above 3,968) are rejected with E3D_EINVAL. Outputs equal
+ * e3d_mlp_predict's bit for bit. Runs on the network's stream and returns
+ * after it drains; the caller must have finished writing x. Calls that
+ * overlap on one GPU share the scratch and race. */
+int e3d_mlp_predict_noalloc(e3d_mlp* mlp, const float* x, long n, float* y);
 /* Average time of one training step's forward and backward passes on a batch
  * (HIP events, reps repetitions) and its matrix-core flops. */
 int e3d_mlp_time_step(e3d_mlp* mlp, int batch, int reps, double* ms, double* flops);
