@@ -38,7 +38,7 @@ __device__ __forceinline__ void central_flux(const Gas& g, const double wl[5], c
     F[4] = 0.5 * (wl[0] * H_L * V_L + wr[0] * H_R * V_R) * norm;
 }
 
-// Roe dissipation with Harten's entropy fix (RoeScheme.cpp:372-479), before the
+// Roe dissipation with Harten's entropy fix (RoeScheme.cpp:183-254), before the
 // beta factor. Returns 0.5*|A|dW*|S|.
 __device__ __forceinline__ void roe_dissipation(const Gas& g, const double wl[5], const double wr[5],
                                                 const double n[3], double norm, double D[5]) {

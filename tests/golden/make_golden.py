@@ -81,6 +81,21 @@ CASES = {
                                  omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5, smooth=1),
     "bump8_ausm1_irs": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=0.5, aoa=2.0, flux=1, order=1, grad=0, lim=1,
                             omega=1e-16, k=5.0, stages=4, iters=12, dumps=[4, 5], stage_iter=5, smooth=1),
+    # supersonic freestream: the farfield BC's supersonic inflow and outflow branches
+    # (FarfieldCellIds.cpp:58-79) and AUSM's |M| >= 1 split-Mach arms (AusmScheme.cpp:143-195, 777-833);
+    # branch_counts() below confirms from the dumped states that each was taken
+    "bump1_roe1_sup": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=1.5, aoa=2.0, flux=0, order=1, grad=0, lim=1,
+                           omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_roe2_gg_vk_sup": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=1.4, aoa=0.0, flux=0, order=2, grad=0, lim=1,
+                                 omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_ausm1_sup": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=1.6, aoa=-1.0, flux=1, order=1, grad=0, lim=1,
+                            omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump1_ausm2_lsq_bj_sup": dict(mesh=(8, 3, 3, (1, 1, 1)), mach=1.3, aoa=3.0, flux=1, order=2, grad=1, lim=0,
+                                   omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump8_roe2_gg_vk_sup": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=1.5, aoa=1.0, flux=0, order=2, grad=0, lim=1,
+                                 omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
+    "bump8_ausm2_gg_vk_sup": dict(mesh=(8, 4, 4, (2, 2, 2)), mach=1.4, aoa=-2.0, flux=1, order=2, grad=0, lim=1,
+                                  omega=1e-16, k=5.0, stages=5, iters=12, dumps=[4, 5], stage_iter=5),
 }
 BIG = {
     # config 1 of the benchmark: 48x16x16x6 tets, first-order Roe, M=0.5, 200 iterations
@@ -148,6 +163,75 @@ def pack(arrays, name, a):
     arrays["s:" + name] = a[idx]
 
 
+def branch_counts(dump, c):
+    """Which branches the reference took, from its dumped arrays.
+
+    Farfield ghosts (FarfieldCellIds.cpp:55-113) after the last dumped
+    iteration: each ghost is classified by its interior state the way the
+    reference does (|u.n/c| > 1, sign of u.n, n = S/area), and the ghost of a
+    supersonic branch must equal that branch's formula exactly (2 W_inf - W_i
+    inflow, W_i outflow). AUSM split Mach numbers (AusmScheme.cpp:143-195,
+    777-833) in the first stage of the stage-dump iteration: the face states
+    W + lim (grad . r) of both sides of every face, M = (u.n)/c, counted per
+    arm (M <= -1, M >= 1) and side."""
+    def load(fn, dt=np.float64):
+        return np.fromfile(os.path.join(dump, fn), dtype=dt)
+    nb = len([f for f in os.listdir(dump) if f.startswith("geom_b") and f.endswith("_counts.i32")])
+    last = max(c["dumps"])
+    out = dict(ff_sup_in=0, ff_sup_out=0, ff_sub_in=0, ff_sub_out=0,
+               mach_L_le_m1=0, mach_L_ge_1=0, mach_R_le_m1=0, mach_R_ge_1=0)
+    g = 1.4
+    for b in range(nb):
+        pre = f"geom_b{b}_"
+        nr, na, nf = load(pre + "counts.i32", np.int32)[:3]
+        S = load(pre + "S.f64").reshape(nf, 3)
+        area = load(pre + "area.f64")
+        f2c = load(pre + "face2cell.i32", np.int32).reshape(nf, 2)
+        off = load(pre + "cell_off.i32", np.int32)
+        c2f = load(pre + "cell2face.i32", np.int32)
+        c2c = load(pre + "cell2cell.i32", np.int32)
+        fs = load(pre + "freestream.f64")
+        bnd = load(pre + "boundaries.i32", np.int32)
+        W = load(f"it{last}_b{b}_W.f64").reshape(5, na)
+        i = 0
+        while i < len(bnd):
+            t, n = bnd[i], bnd[i + 1]
+            ids = bnd[i + 2:i + 2 + n]
+            i += 2 + n
+            if t != 1:
+                continue
+            for gh in ids:
+                ic, f = c2c[off[gh]], c2f[off[gh]]
+                nx, ny, nz = S[f, 0] / area[f], S[f, 1] / area[f], S[f, 2] / area[f]
+                wi = W[:, ic]
+                cc = np.sqrt(g * wi[4] / wi[0])
+                un1 = wi[1] * nx + wi[2] * ny + wi[3] * nz
+                if abs(un1 / cc) > 1.0:
+                    want = 2.0 * fs - wi if un1 < 0.0 else wi
+                    if not np.array_equal(W[:, gh], want):
+                        raise RuntimeError(f"farfield ghost {gh} of block {b} is not its supersonic branch's value")
+                    out["ff_sup_in" if un1 < 0.0 else "ff_sup_out"] += 1
+                else:
+                    out["ff_sub_in" if un1 < 0.0 else "ff_sub_out"] += 1
+        tag = f"it{c['stage_iter']}s0_b{b}_"
+        if c["stage_iter"] < 0 or not os.path.exists(os.path.join(dump, tag + "W.f64")):
+            continue
+        Ws = load(tag + "W.f64").reshape(5, na)
+        lim = load(tag + "lim.f64").reshape(5, na)
+        grad = load(tag + "grad.f64").reshape(5, 3, na)
+        rL = load(pre + "rL.f64").reshape(nf, 3)
+        rR = load(pre + "rR.f64").reshape(nf, 3)
+        n = S / area[:, None]
+        for side, cells, r in (("L", f2c[:, 0], rL), ("R", f2c[:, 1], rR)):
+            w = Ws[:, cells].copy()
+            for d in range(3):
+                w += lim[:, cells] * grad[:, d, cells] * r[:, d]
+            M = (w[1] * n[:, 0] + w[2] * n[:, 1] + w[3] * n[:, 2]) / np.sqrt(g * w[4] / w[0])
+            out[f"mach_{side}_le_m1"] += int((M <= -1.0).sum())
+            out[f"mach_{side}_ge_1"] += int((M >= 1.0).sum())
+    return out
+
+
 def run_case(name, c, outdir):
     import euler3d_amd as E
     tmp = tempfile.mkdtemp(prefix=f"golden_{name}_")
@@ -170,6 +254,7 @@ def run_case(name, c, outdir):
         r = subprocess.run(args, cwd=tmp, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: ref_dump failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        meta["branches"] = branch_counts(dump, c)
         arrays = dict(extra)
         keep_geom = c["mesh"] != (48, 16, 16, (1, 1, 1))
         for fn in sorted(os.listdir(dump)):
@@ -200,7 +285,7 @@ def run_case(name, c, outdir):
         arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
         path = os.path.join(outdir, name + ".npz")
         np.savez_compressed(path, **arrays)
-        print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB, {len(arrays)} arrays")
+        print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB, {len(arrays)} arrays, branches {meta['branches']}")
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
