@@ -112,24 +112,26 @@ def test_batch_criterion_stops_instance_and_freezes_it():
 
 
 def test_batch_tile_order_does_not_change_results():
-    meshes = [mk() for mk, _, _ in CASES[:2]]
-    insts = [dict(mach=m, aoa_deg=a, criterion=-15.0, max_iter=20) for _, m, a in CASES[:2]]
-    a = batch_context(meshes, insts)
-    b = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
-    b.set_tile_order(E.TILES_LINEAR)
-    c = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
-    c.set_tile_order(E.TILES_XCD_CONTIGUOUS)
-    for x in (a, b, c):
-        x.batch_run(8, use_graph=True)
-    for other in (b, c):
+    """Every tile order, the automatic one included, gives the same bits in a
+    batch where one instance stops early (its tiles are skipped from then on)."""
+    insts = [dict(mach=m, aoa_deg=a, criterion=-15.0, max_iter=n) for (_, m, a), n in zip(CASES[:2], (5, 20))]
+    ctxs = {}
+    for order in (E.TILES_XCD_CHUNKS, E.TILES_LINEAR, E.TILES_XCD_CONTIGUOUS, E.TILES_AUTO):
+        x = batch_context([mk() for mk, _, _ in CASES[:2]], insts)
+        x.set_tile_order(order)
+        done, status = x.batch_run(8, use_graph=True)
+        assert status[0]["stopped"] == 1 and status[0]["iterations"] == 5 and status[1]["iterations"] == 8
+        ctxs[order] = x
+    a = ctxs[E.TILES_XCD_CHUNKS]
+    for order, other in ctxs.items():
         for blk in range(len(a.sizes)):
-            assert (a.get(blk, E.ARR_W).view(np.int64) == other.get(blk, E.ARR_W).view(np.int64)).all()
-        assert a.batch_history(1, 20) == other.batch_history(1, 20)
+            assert (a.get(blk, E.ARR_W).view(np.int64) == other.get(blk, E.ARR_W).view(np.int64)).all(), order
+        for i in range(2):
+            assert a.batch_history(i, 20) == other.batch_history(i, 20), order
     for bad in (3, -2):
         with pytest.raises(E.E3DError):
             a.set_tile_order(bad)
-    a.set_tile_order(E.TILES_AUTO)
-    for x in (a, b, c):
+    for x in ctxs.values():
         x.close()
 
 
