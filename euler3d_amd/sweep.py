@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import sys
 import time
 
@@ -82,8 +83,10 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
     wall = time.perf_counter() - t1
     cell_stages = 0
     for c, s in zip(cases, status):
+        # a non-finite residual norm stops an instance (the reference would run on to max_iter)
+        diverged = not all(math.isfinite(x) for x in s["last"]["rms"])
         c.update(iterations=s["iterations"], stopped=s["stopped"], stop_seconds=s["stop_seconds"],
-                 drop=s["last"]["drop"][0], cl=s["last"]["cl"], cd=s["last"]["cd"],
+                 drop=s["last"]["drop"][0], cl=s["last"]["cl"], cd=s["last"]["cd"], diverged=diverged,
                  reached_criterion=bool(s["last"]["drop"][0] <= criterion))
         cell_stages += c["cells"] * SCHEME["stages"] * s["iterations"]
     ctx.close()
@@ -91,7 +94,8 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
         m.close()
     return {"instances": cases, "wall_seconds": wall, "iterations_launched": done,
             "cells": sum(c["cells"] for c in cases), "mcell_rk_stage_per_s": cell_stages / wall / 1e6,
-            "reached_criterion": sum(c["reached_criterion"] for c in cases), "tile_order": tile_order,
+            "reached_criterion": sum(c["reached_criterion"] for c in cases),
+            "diverged": [c["seed"] for c in cases if c["diverged"]], "tile_order": tile_order,
             "mesh_seconds": t_mesh, "tune_seconds": t_tune, "max_iter": max_iter, "criterion": criterion, "graph": use_graph}
 
 
@@ -109,7 +113,8 @@ def main(argv=None):
     r = run(a.instances, a.max_iter, a.criterion, a.scale, not a.no_graph, not a.no_autotune, a.seed0, log=sys.stderr)
     print(f"[sweep] {r['cells']} cells in {len(r['instances'])} instances: {r['iterations_launched']} iterations "
           f"in {r['wall_seconds']:.2f}s = {r['mcell_rk_stage_per_s']:.0f} Mcell-RK-stage/s; "
-          f"{r['reached_criterion']} reached a {-a.criterion:g}-decade drop", file=sys.stderr, flush=True)
+          f"{r['reached_criterion']} reached a {-a.criterion:g}-decade drop; diverged (non-finite residual): "
+          f"seeds {r['diverged']}", file=sys.stderr, flush=True)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(r, f, indent=1)
