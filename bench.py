@@ -193,32 +193,36 @@ def reference_baseline(sc, nx=24, iters=40):
     return out
 
 
+STREAM_MIXES = [(1, 0), (7, 1), (3, 1), (2, 1), (1, 1), (1, 2), (0, 1)]  # planes read : planes written
+
+
 def stream_bandwidth(gib=4, reps=10):
-    """Achievable HBM bandwidth on this GPU for three traffic mixes, with
-    PyTorch's own kernels: copy (half reads, half writes), sum (reads) and fill
-    (writes) of a gib-GiB FP64 array, HIP events around reps launches."""
-    import torch
-    dev = torch.device("cuda:0")
-    n = gib * (1 << 30) // 8
-    a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
-    b = torch.empty_like(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    out = {}
-    for name, fn, nbytes in (("copy", lambda: b.copy_(a), 2 * n * 8), ("read", lambda: a.sum(), n * 8),
-                             ("write", lambda: b.fill_(2.0), n * 8)):
-        for _ in range(3):
-            fn()
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        out[name] = {"ms": ms, "GB/s": nbytes / ms / 1e6}
-        print(f"[bench] {name}: {ms:.3f} ms per {gib} GiB launch = {nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr,
-              flush=True)
+    """Achievable HBM bandwidth on this GPU by traffic mix, with the library's
+    own streaming kernel (e3d_stream_bandwidth: 16-byte nontemporal loads and
+    stores, consecutive lanes on consecutive addresses) moving gib GiB per
+    launch, HIP events around reps launches after 3 warmup launches."""
+    import ctypes
+
+    import euler3d_amd as E
+    out = []
+    for r, w in STREAM_MIXES:
+        g = ctypes.c_double()
+        E._check(E.lib().e3d_stream_bandwidth(0, gib << 30, r, w, reps, ctypes.byref(g)))
+        out.append({"reads": r, "writes": w, "write_fraction": w / (r + w), "GB/s": g.value,
+                    "frac_of_peak": g.value / HBM_PEAK_GBS})
+        print(f"[bench] stream {r}:{w} (writes {w / (r + w):.3f}): {g.value:.0f} GB/s = "
+              f"{g.value / HBM_PEAK_GBS:.3f} of peak", file=sys.stderr, flush=True)
     return out
+
+
+def stream_rate(table, write_fraction):
+    """Streaming rate of a traffic mix: linear in the write fraction between the
+    two measured mixes around it."""
+    pts = sorted((m["write_fraction"], m["GB/s"]) for m in table)
+    for (x0, y0), (x1, y1) in zip(pts, pts[1:]):
+        if x0 <= write_fraction <= x1:
+            return y0 + (y1 - y0) * (write_fraction - x0) / (x1 - x0)
+    return pts[-1][1]
 
 
 def main():
@@ -239,13 +243,15 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="advance through the batch API with one instance: stop test on the device and a "
                          "hipGraph of two iterations replayed (no per-kernel timing)")
+    ap.add_argument("--tile-cells", type=int, choices=[64, 128, 256], default=256,
+                    help="cells per workgroup of the stage kernels (results are the same bit for bit)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy: the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--stream-bw", action="store_true",
-                    help="only measure the achievable HBM bandwidth of copy / read / write streams and exit")
+                    help="only measure the achievable HBM bandwidth of streams of 7 read/write mixes and exit")
     a = ap.parse_args()
     if a.stream_bw:
         print(json.dumps({"stream_bandwidth": stream_bandwidth(), "peak_GB/s": HBM_PEAK_GBS}))
@@ -277,6 +283,7 @@ def main():
         ctx.upload_batch([mesh], [dict(mach=sc["mach"], aoa_deg=sc["aoa"], criterion=-1e9, max_iter=10 ** 9)])
     else:
         ctx.upload(mesh)
+    ctx.set_tile_cells(a.tile_cells)
     ctx.init_freestream()
     t_up = time.perf_counter() - t_up
     print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s, uploaded "
@@ -350,8 +357,9 @@ def main():
                    "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages,
                    "cell_numbering": "scrambled" if a.brick < 0 else f"brick {a.brick}",
                    "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate",
-                   "tile_order": "automatic by mesh size (XCD-contiguous below 1024 tiles of 256 cells, "
-                                 "runs of 16 tiles dealt to the XCDs above)"},
+                   "tile_cells": a.tile_cells,
+                   "tile_order": "automatic by mesh size (XCD-contiguous below 262,144 cells, "
+                                 "runs of 4096 cells dealt to the XCDs above)"},
         "model_bytes_per_cell_stage": model_bytes,
         "hbm_roof_frac_overall": value / (HBM_PEAK_GBS * 1e9 / model_bytes / 1e6),
         "roofline": rooflines[dominant] if dominant else None,

@@ -125,12 +125,14 @@ def lib():
             "e3d_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
             "e3d_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_sync": (C.c_int, [C.c_void_p]),
+            "e3d_stream_bandwidth": (C.c_int, [C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, _dp]),
             "e3d_upload_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(Instance), C.c_int]),
             "e3d_batch_size": (C.c_int, [C.c_void_p]),
             "e3d_batch_blocks": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
             "e3d_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(BatchStatus)]),
             "e3d_batch_history": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor), C.c_int]),
             "e3d_set_tile_order": (C.c_int, [C.c_void_p, C.c_int]),
+            "e3d_set_tile_cells": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_set_ordering": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
                                          C.POINTER(C.c_void_p)]),
@@ -328,6 +330,10 @@ class Context:
     def set_tile_order(self, order):
         """TILES_AUTO (default), TILES_LINEAR, TILES_XCD_CONTIGUOUS or TILES_XCD_CHUNKS; addresses only."""
         self._c(lib().e3d_set_tile_order(self._h, int(order)))
+
+    def set_tile_cells(self, cells):
+        """Cells per workgroup of the stage kernels: 64, 128 or 256 (default); addresses only."""
+        self._c(lib().e3d_set_tile_cells(self._h, int(cells)))
 
     def set_ordering(self, mode):
         """Memory order of the next upload: ORDER_REFERENCE, ORDER_RCM or ORDER_MORTON."""

@@ -33,14 +33,14 @@ void launch_smooth(const Range&, const DMesh&, const DState&, hipStream_t);
 void launch_block_monitors(int, int, const int*, const int*, const int*, const int*, const int*, const int*,
                            const double*, const DMesh&, const double*, const double*, const double*, double*, double*,
                            hipStream_t);
-void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, hipStream_t);
+void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, int, hipStream_t);
 void launch_batch_reduce(int, const int*, const double*, const void*, int*, double*, int*, int*, void*, int, int*,
                          long long*, hipStream_t);
 void launch_clock_mark(long long*, hipStream_t);
 void launch_batch_freeze(const DMesh&, const int*, const int*, const double*, double*, hipStream_t);
 
 void launch_flux_update(const DMesh&, const DState&, int, bool, bool, bool, bool, const Gas&, double, double, double,
-                        double, hipStream_t);
+                        double, int, hipStream_t);
 }  // namespace e3d
 
 using namespace e3d;
@@ -103,6 +103,7 @@ struct e3d_ctx {
     double* graph_W = nullptr;   // W buffer the graph was captured with
     bool capturing = false;
     int tile_pref = E3D_TILES_AUTO;       // tile order asked for (e3d_set_tile_order)
+    int tile_cells = 256;                 // cells per workgroup of the stage kernels (e3d_set_tile_cells)
     int tile_map = E3D_TILES_XCD_CHUNKS;  // tile order in use (resolve_tile_order)
     int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
     // per block: device position of each real cell and face (empty: the reference's numbering)
@@ -147,14 +148,13 @@ void drop_graph(e3d_ctx* c) {
 }
 
 // The tile order in use: the one asked for, or by size (E3D_TILES_AUTO): a
-// mesh of fewer than kAutoChunkTiles tiles keeps each XCD on one contiguous
-// range (runs of 16 tiles would leave some XCDs a run short), a larger one
+// mesh of fewer than kAutoChunkCells cells keeps each XCD on one contiguous
+// range (runs of 4096 cells would leave some XCDs a run short), a larger one
 // deals the runs to the XCDs in turn.
-constexpr int kAutoChunkTiles = 1024;
+constexpr int kAutoChunkCells = 1024 * 256;
 void resolve_tile_order(e3d_ctx* c) {
-    const int tiles = c->m.NRp / kWG;
     c->tile_map = c->tile_pref != E3D_TILES_AUTO ? c->tile_pref
-                  : tiles < kAutoChunkTiles      ? E3D_TILES_XCD_CONTIGUOUS
+                  : c->m.NRp < kAutoChunkCells   ? E3D_TILES_XCD_CONTIGUOUS
                                                  : E3D_TILES_XCD_CHUNKS;
     c->m.tile_map = c->tile_map;
 }
@@ -484,12 +484,11 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     std::vector<double> vol(NA, 1.0), lsqw((size_t)nslot * 3 * NR, 0.0);
     std::vector<int> wg_block(NR / kWG, 0), tile0(nb), ntile(nb), wall_off(nb + 1, 0), wall_ghost, wall_face;
     std::vector<int> itf_dst, itf_org;
-    // per-slot face geometry of the fused path
+    // per-slot face geometry of the fused path: S oriented away from the cell
+    // and the cell's r vector, each as an (x, y) pair plane and a z plane
     const size_t NS = (size_t)nslot * NR;
-    std::vector<double> sSR(6 * NS, 0.0), sNA(4 * NS, 0.0), g_r(3 * (size_t)NG, 0.0);
-    for (size_t q = 0; q < NS; ++q) sNA[2 * (NS + q) + 1] = 1.0;  // |S| of padding slots
-    // component j (0..5 of sSR, 0..3 of sNA) of slot k of cell gc: pair j/2, half j%2
-    auto pair_at = [&](int j, int k, int gcell) { return 2 * (((size_t)(j / 2) * nslot + k) * NR + gcell) + j % 2; };
+    std::vector<double> sSxy(2 * NS, 0.0), sSz(NS, 0.0), sRxy(2 * NS, 0.0), sRz(NS, 0.0), g_r(3 * (size_t)NG, 0.0),
+        g_n(3 * (size_t)NG, 0.0);
     std::vector<int> mate(NS, 0);
     std::vector<int> g_blk(NG, 0);
     for (int b = 0; b < nb; ++b) {
@@ -526,19 +525,20 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
                     fid[(size_t)k * NR + gc] = global_face(c, b, f) | (right ? (int)0x80000000u : 0);
                     for (int d = 0; d < 3; ++d)
                         lsqw[(size_t)(k * 3 + d) * NR + gc] = B.lsqw[3 * (size_t)(B.cell_off[q] + k) + d];
-                    // geometry copies: S, own-side r, S/|S| and |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz)
-                    // as in FluxScheme.cpp:105-112 (equal to the stored face area)
-                    const double sx = B.S[3 * f], sy = B.S[3 * f + 1], sz = B.S[3 * f + 2];
-                    const double norm = std::sqrt(sx * sx + sy * sy + sz * sz);
+                    // geometry copies: S oriented away from this cell, as the gradient uses it
+                    // (GreenGauss.cpp:104-118), and the r vector on this cell's side
                     const double* rv = right ? &B.rR[3 * f] : &B.rL[3 * f];
-                    const double sv[3] = {sx, sy, sz};
+                    const size_t q2 = (size_t)k * NR + gc;
                     for (int d = 0; d < 3; ++d) {
-                        // oriented away from this cell, as the gradient uses it (GreenGauss.cpp:104-118)
-                        sSR[pair_at(d, k, gc)] = right ? sv[d] * -1.0 : sv[d];
-                        sSR[pair_at(3 + d, k, gc)] = rv[d];
-                        sNA[pair_at(d, k, gc)] = sv[d] / norm;
+                        const double sd = right ? B.S[3 * f + d] * -1.0 : B.S[3 * f + d];
+                        if (d < 2) {
+                            sSxy[2 * q2 + d] = sd;
+                            sRxy[2 * q2 + d] = rv[d];
+                        } else {
+                            sSz[q2] = sd;
+                            sRz[q2] = rv[d];
+                        }
                     }
-                    sNA[pair_at(3, k, gc)] = norm;
                     // face-state record of the other cell on this face
                     size_t rec;
                     if (other < B.nreal) {
@@ -570,7 +570,15 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
                 const int gi = c->goff[b] + (gl - B.nreal);
                 g_in[gi] = global_cell(c, b, B.cell2cell[B.cell_off[gl]]);
                 g_face[gi] = global_face(c, b, B.cell2face[B.cell_off[gl]]);
-                for (int d = 0; d < 3; ++d) g_r[(size_t)d * NG + gi] = B.rR[3 * (size_t)B.cell2face[B.cell_off[gl]] + d];
+                const int gf = B.cell2face[B.cell_off[gl]];
+                // unit normal of the ghost's face: S / |S| with |S| = sqrt(Sx*Sx + Sy*Sy + Sz*Sz), as
+                // FluxScheme.cpp:105-112 forms it (equal to S / face area, MetricsInitializer.cpp:546-558)
+                const double sx = B.S[3 * gf], sy = B.S[3 * gf + 1], sz = B.S[3 * gf + 2];
+                const double norm = std::sqrt(sx * sx + sy * sy + sz * sz);
+                for (int d = 0; d < 3; ++d) {
+                    g_r[(size_t)d * NG + gi] = B.rR[3 * (size_t)gf + d];
+                    g_n[(size_t)d * NG + gi] = B.S[3 * (size_t)gf + d] / norm;
+                }
                 int kind = 0;
                 if (real_bnd) {
                     kind |= GK_REFLECT;
@@ -612,7 +620,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     const int* cnf;
     (void)cnf;
     int *d_nf, *d_nbr, *d_fid, *d_gin, *d_gface, *d_gkind, *d_wgb, *d_mate;
-    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw, *d_sSR, *d_sNA, *d_gr;
+    double *d_S, *d_area, *d_rL, *d_rR, *d_vol, *d_lsqw, *d_sSxy, *d_sSz, *d_sRxy, *d_sRz, *d_gr, *d_gn;
     if ((rc = upload(c, &d_nf, nf)) || (rc = upload(c, &d_nbr, nbr)) || (rc = upload(c, &d_fid, fid)) ||
         (rc = upload(c, &d_gin, g_in)) || (rc = upload(c, &d_gface, g_face)) || (rc = upload(c, &d_gkind, g_kind)) ||
         (rc = upload(c, &d_wgb, wg_block)) || (rc = upload(c, &d_S, S)) || (rc = upload(c, &d_area, area)) ||
@@ -622,7 +630,8 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         (rc = upload(c, &c->d_ntile, ntile)) || (rc = upload(c, &c->d_nreal, c->nreal)) ||
         (rc = upload(c, &c->d_wall_off, wall_off)) || (rc = upload(c, &c->d_wall_ghost, wall_ghost)) ||
         (rc = upload(c, &c->d_wall_face, wall_face)) || (rc = upload(c, &d_mate, mate)) ||
-        (rc = upload(c, &d_sSR, sSR)) || (rc = upload(c, &d_sNA, sNA)) || (rc = upload(c, &d_gr, g_r)) || (rc = upload(c, &c->d_blk_inst, c->blk_inst)) ||
+        (rc = upload(c, &d_sSxy, sSxy)) || (rc = upload(c, &d_sSz, sSz)) || (rc = upload(c, &d_sRxy, sRxy)) ||
+        (rc = upload(c, &d_sRz, sRz)) || (rc = upload(c, &d_gr, g_r)) || (rc = upload(c, &d_gn, g_n)) || (rc = upload(c, &c->d_blk_inst, c->blk_inst)) ||
         (rc = upload(c, &c->d_g_blk, g_blk)) || (rc = upload(c, &c->d_inst_b0, c->inst_b0)) ||
         (rc = dalloc(c, &c->d_inst_fs, 5 * (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_blk_qcmac, (size_t)nb)) ||
         (rc = dalloc(c, &c->d_active, (size_t)c->n_inst)) || (rc = dalloc(c, &c->d_frozen, (size_t)c->n_inst)) ||
@@ -634,9 +643,11 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     }
     m.nf = d_nf; m.nbr = d_nbr; m.fid = d_fid; m.S = d_S; m.area = d_area; m.rL = d_rL; m.rR = d_rR;
     m.vol = d_vol; m.lsqw = d_lsqw; m.g_in = d_gin; m.g_face = d_gface; m.g_kind = d_gkind; m.wg_block = d_wgb;
-    m.sSR = reinterpret_cast<const double2*>(d_sSR);
-    m.sNA = reinterpret_cast<const double2*>(d_sNA);
-    m.mate = d_mate; m.g_r = d_gr;
+    m.sSxy = reinterpret_cast<const double2*>(d_sSxy);
+    m.sSz = d_sSz;
+    m.sRxy = reinterpret_cast<const double2*>(d_sRxy);
+    m.sRz = d_sRz;
+    m.mate = d_mate; m.g_r = d_gr; m.g_n = d_gn;
     m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr;
     resolve_tile_order(c);
     DState& s = c->s;
@@ -644,7 +655,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     if ((rc = dalloc(c, &s.W, 5 * NA)) || (rc = dalloc(c, &s.Wn, 5 * NA)) || (rc = dalloc(c, &s.W0, 5 * NR)) ||
         (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
         (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.Wf, 5 * (NS + NG))) || (rc = dalloc(c, &s.dt, NR)) ||
-        (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / kWG + 1))) ||
+        (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / 64 + 1))) ||
         (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb)) ||
         (rc = dalloc(c, &c->d_mpart, 10 * (size_t)nb * c->mon_split))) {
         free_device(c);
@@ -995,12 +1006,12 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
             if (order2) {
                 ev_begin(c, KI_INTERP, &ev);
                 launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, k * k * k, p.omega,
-                                    c->stream);
+                                    c->tile_cells, c->stream);
                 ev_end(c, ev);
             }
             ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
             launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, st == 0, c->gas,
-                               c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->stream);
+                               c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->tile_cells, c->stream);
             ev_end(c, ev);
             if ((rc = launch_status(c))) return rc;
             std::swap(c->s.W, c->s.Wn);
@@ -1065,12 +1076,12 @@ static int batch_iteration(e3d_ctx* c) {
         if (order2) {
             ev_begin(c, KI_INTERP, &ev);
             launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, p.k * p.k * p.k, p.omega,
-                                c->stream);
+                                c->tile_cells, c->stream);
             ev_end(c, ev);
         }
         ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
         launch_flux_update(c->m, c->s, p.flux, order2, c->beta[st] != 0.0, st == p.stages - 1, st == 0, c->gas,
-                           c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->stream);
+                           c->beta[st], c->alpha[st], p.cfl, 1. / (p.gamma - 1.0), c->tile_cells, c->stream);
         ev_end(c, ev);
         if ((rc = launch_status(c))) return rc;
         std::swap(c->s.W, c->s.Wn);
@@ -1205,6 +1216,15 @@ int e3d_set_tile_order(e3d_ctx* c, int order) {
         return set_error(c, E3D_EINVAL, "e3d_set_tile_order: order must be -1, 0, 1 or 2");
     c->tile_pref = order;
     resolve_tile_order(c);
+    drop_graph(c);
+    return E3D_OK;
+}
+
+int e3d_set_tile_cells(e3d_ctx* c, int cells) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    if (cells != 64 && cells != 128 && cells != 256)
+        return set_error(c, E3D_EINVAL, "e3d_set_tile_cells: cells must be 64, 128 or 256");
+    c->tile_cells = cells;
     drop_graph(c);
     return E3D_OK;
 }

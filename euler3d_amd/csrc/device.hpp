@@ -4,7 +4,7 @@
 
 namespace e3d {
 
-constexpr int kWG = 256;          // threads per workgroup; block real ranges are padded to this
+constexpr int kWG = 256;          // threads per workgroup of the granular kernels; block real ranges are padded to this
 constexpr int kMaxSlots = 6;      // faces per cell: tet 4, pyramid 5, hexahedron 6
 
 // Ghost kind bits.
@@ -33,15 +33,17 @@ struct DMesh {
     const int* g_in;      // [NG] interior (global) of each ghost
     const int* g_face;    // [NG] face of each ghost
     const int* g_kind;    // [NG] GK_* bits
-    const int* wg_block;  // [NRp/kWG] block of each workgroup-sized tile
-    // Per-slot copies of face geometry for the fused path in 16-byte pairs,
-    // slot-major so a workgroup streams them coalesced: pair p of slot k of
-    // cell c at (p*nslot + k)*NRp + c.
-    const double2* sSR;   // [3][nslot][NRp] (Sx, Sy), (Sz, rx), (ry, rz): area vector S of the face, negated
-                          // when this cell is its right cell, and the face's r vector on this cell's side
-    const double2* sNA;   // [2][nslot][NRp] (nx, ny), (nz, |S|): unit normal S/|S| of the face as stored, |S|
+    const int* wg_block;  // [NRp/kWG] block of each kWG-cell range
+    // Per-slot copies of face geometry for the fused path, slot-major so a
+    // workgroup streams them coalesced: entry of slot k of cell c at k*NRp + c,
+    // each vector as an (x, y) pair plane and a z plane.
+    const double2* sSxy;  // [nslot][NRp] area vector S of the face, negated when this cell is its right cell
+    const double* sSz;
+    const double2* sRxy;  // [nslot][NRp] the face's r vector on this cell's side
+    const double* sRz;
     const int* mate;      // [nslot][NRp] face-state record of the other cell of the face; bit 31: this cell is right
     const double* g_r;    // [3][NG] r vector of each ghost on its face (the reflected right vector)
+    const double* g_n;    // [3][NG] unit normal S/|S| of each ghost's face (S as stored: out of the interior cell)
     // Instances: a context may hold a batch of independent meshes, each with its
     // own flow conditions; a single mesh is instance 0.
     const int* blk_inst;  // [nb] instance of each block
@@ -73,7 +75,7 @@ struct DState {
     double* Wf;           // [5 * (nslot*NRp + NG)] reconstructed face states (fused path), three planes
     double* dt;           // [NRp]
     double* specrad;      // [NRp]
-    double* part;         // [ntiles][8] per-tile partial sums of the residual monitor
+    double* part;         // [NRp/64][8] per-wavefront partial sums of the residual monitor
     double* jac;          // [10][NRp] Jacobi values of the residual smoother (allocated on first use)
 };
 

@@ -11,11 +11,20 @@
 //              and the ghost BCs, into the second state buffer. On stage 0
 //              also saveW0, the spectral radius and the local time step.
 //
-// Face geometry is stored per (cell, face) slot so it streams coalesced; the
-// only gathers are the neighbour states in k_interp and one face-state record
-// per face in k_flux. Every value equals the granular path's bit for bit: the
-// per-slot copies hold the same doubles, and the reconstruction, gradient,
-// limiter and flux expressions are evaluated in the reference's order.
+// Face geometry is stored per (cell, face) slot so it streams coalesced: the
+// face's area vector S, oriented away from the cell, and the r vector on the
+// cell's side. k_flux rebuilds the unit normal and |S| from S with the
+// reference's own expressions (FluxScheme.cpp:105-112), which give the same
+// doubles as the stored face normal and area. The only gathers are the
+// neighbour states in k_interp and one face-state record per face in k_flux.
+// Every value equals the granular path's bit for bit: the per-slot copies hold
+// the same doubles, and the reconstruction, gradient, limiter and flux
+// expressions are evaluated in the reference's order.
+//
+// Both kernels are instantiated for tiles of 64, 128 or 256 cells per
+// workgroup (e3d_set_tile_cells); a thread owns one cell either way, so the
+// tile size moves no result bit. The last stage's residual-norm partial sums
+// are formed per wavefront (64 cells) for the same reason.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -74,20 +83,20 @@ __device__ __forceinline__ void store_w32(double* W, int NA, int x, const double
 #pragma unroll
     for (int v = 0; v < 5; ++v) st32_stream(W + (size_t)v * NA, x, w[v]);
 }
-// Pair p of slot k of cell c in a per-slot pair array.
-__device__ __forceinline__ double2 pair32(const double2* a, int p, int nslot, int NRp, int k, int c) {
-    return ld32(a + ((size_t)p * nslot + k) * NRp, c);
+// S (oriented away from this cell) and r vector of slot k of cell c.
+__device__ __forceinline__ void slot_S(const DMesh& m, int k, int c, double S[3]) {
+    const size_t o = (size_t)k * m.NRp;
+    const double2 xy = ld32_stream(m.sSxy + o, c);
+    S[0] = xy.x;
+    S[1] = xy.y;
+    S[2] = ld32_stream(m.sSz + o, c);
 }
-__device__ __forceinline__ double2 pair32_stream(const double2* a, int p, int nslot, int NRp, int k, int c) {
-    return ld32_stream(a + ((size_t)p * nslot + k) * NRp, c);
-}
-// Unit normal and |S| of slot k of cell c.
-__device__ __forceinline__ double slot_normal(const DMesh& m, int k, int c, double n[3]) {
-    const double2 a = pair32_stream(m.sNA, 0, m.nslot, m.NRp, k, c), b = pair32_stream(m.sNA, 1, m.nslot, m.NRp, k, c);
-    n[0] = a.x;
-    n[1] = a.y;
-    n[2] = b.x;
-    return b.y;
+__device__ __forceinline__ void slot_r(const DMesh& m, int k, int c, double r[3]) {
+    const size_t o = (size_t)k * m.NRp;
+    const double2 xy = ld32_stream(m.sRxy + o, c);
+    r[0] = xy.x;
+    r[1] = xy.y;
+    r[2] = ld32_stream(m.sRz + o, c);
 }
 __device__ __forceinline__ void load_w32(const double* W, int NA, int x, double w[5]) {
 #pragma unroll
@@ -163,7 +172,7 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
     const int gi = nb - NRp;
     const bool reflect = m.g_kind[gi] & GK_REFLECT;
     double n[3], rg[3];
-    (void)slot_normal(m, k, c, n);  // S / area
+    load3(m.g_n, m.NG, gi, n);  // S / area
     load3(m.g_r, m.NG, gi, rg);
     const size_t nrec = rec_count(m), rec = ghost_record(ns, NRp, gi);
 #pragma unroll
@@ -195,11 +204,11 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
     }
 }
 
-template <int GRAD, bool VENKAT, int MAXS>
-__global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
-    const int tile = tile_of(m, ntiles);
-    if (tile >= ntiles || !instance_active(m, tile_instance(m, tile))) return;
-    const int c = tile * kWG + threadIdx.x;
+template <int GRAD, bool VENKAT, int MAXS, int WG>
+__global__ void __launch_bounds__(WG) k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
+    const int tile = tile_of<WG>(m, ntiles);
+    if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
+    const int c = tile * WG + threadIdx.x;
     const int nf = ld32_stream(m.nf, c);
     if (nf == 0) return;
     const int NRp = m.NRp, ns = m.nslot;
@@ -210,7 +219,7 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
     // The neighbour ids of all slots are loaded first (slots k >= nf hold
     // in-bounds padding and are not used), so each face then waits for one
     // memory round trip (its neighbour's state and its geometry), not two.
-    __shared__ double r_lds[MAXS * 3][kWG];  // r vectors of the cell's faces, for passes 2 and 3
+    __shared__ double r_lds[MAXS * 3][WG];  // r vectors of the cell's faces, for passes 2 and 3
     double g[15], mx[5], mn[5];
     unsigned ghost_slots = 0;  // bit k: the neighbour across face slot k is a ghost
     int nbv[MAXS];
@@ -223,23 +232,19 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
-            double wn[5], geo[3];
+            double wn[5], geo[3], r[3];
             load_w32(s.W, m.NA, nbv[k], wn);
             // S oriented away from this cell (Green-Gauss) or the least-squares
             // weights; the r vector goes to LDS for the limiter and reconstruction
-            const double2 p1 = pair32_stream(m.sSR, 1, ns, NRp, k, c), p2 = pair32_stream(m.sSR, 2, ns, NRp, k, c);
             if (GRAD == 0) {
-                const double2 p0 = pair32_stream(m.sSR, 0, ns, NRp, k, c);
-                geo[0] = p0.x;
-                geo[1] = p0.y;
-                geo[2] = p1.x;
+                slot_S(m, k, c, geo);
             } else {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) geo[d] = ld32_stream(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
             }
-            r_lds[k * 3 + 0][threadIdx.x] = p1.y;
-            r_lds[k * 3 + 1][threadIdx.x] = p2.x;
-            r_lds[k * 3 + 2][threadIdx.x] = p2.y;
+            slot_r(m, k, c, r);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) r_lds[k * 3 + d][threadIdx.x] = r[d];
             if (nbv[k] >= NRp) ghost_slots |= 1u << k;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -304,15 +309,15 @@ __global__ void __launch_bounds__(kWG) k_interp(DMesh m, DState s, double k3, do
 // the stages: saveW0 (Solver.cpp:161-210) and the spectral radius and local
 // time step (Timestep.cpp:67-146), all from the cell's own state at the start
 // of the iteration, with the spectral radius summed in the same face loop.
-template <int FLUX, bool ORDER2, bool DISS, bool LAST, bool FIRST, int MAXS>
-__global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double beta, double alpha, double cfl,
-                                              double inv_gm1, int ntiles) {
-    const int tile = tile_of(m, ntiles);
+template <int FLUX, bool ORDER2, bool DISS, bool LAST, bool FIRST, int MAXS, int WG>
+__global__ void __launch_bounds__(WG) k_flux(DMesh m, DState s, Gas gas, double beta, double alpha, double cfl,
+                                             double inv_gm1, int ntiles) {
+    const int tile = tile_of<WG>(m, ntiles);
     if (tile >= ntiles) return;
-    const int inst = tile_instance(m, tile);
+    const int inst = cell_instance(m, tile * WG);
     if (!instance_active(m, inst)) return;
     const double* fs = m.inst_fs + 5 * inst;
-    const int c = tile * kWG + threadIdx.x;
+    const int c = tile * WG + threadIdx.x;
     const int nf = ld32_stream(m.nf, c);
     const int NRp = m.NRp, ns = m.nslot;
     const size_t ghost0 = (size_t)ns * NRp;
@@ -336,8 +341,13 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 const int mt = mtv[k];
                 const bool right = mt < 0;
                 const size_t rec = (size_t)(mt & 0x7fffffff);
-                double n[3], own[5], oth[5];
-                const double norm = slot_normal(m, k, c, n);
+                // the face's S as stored (the slot holds it negated on the right
+                // cell: negation is exact), its unit normal and |S|
+                double Sv[3], n[3], norm;
+                slot_S(m, k, c, Sv);
+                const double sgn = right ? -1.0 : 1.0;  // exact: S * -1 is S negated
+                unit_normal(Sv[0] * sgn, Sv[1] * sgn, Sv[2] * sgn, n, norm);
+                double own[5], oth[5];
                 if (FIRST) sr += (fabs((wc[1]) * n[0] + (wc[2]) * n[1] + (wc[3]) * n[2]) + (c_center)) * norm;
                 if (ORDER2) {
                     load_own_rec(s.Wf, rec_count(m), NRp, k, c, own);
@@ -418,7 +428,7 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 double gw[5];
                 if (kind & GK_BC_MASK) {
                     double n[3];
-                    (void)slot_normal(m, k, c, n);  // S / area
+                    load3(m.g_n, m.NG, gi, n);  // S / area
                     if ((kind & GK_BC_MASK) == GK_SLIP) ghost_slip(wnew, n, gw);
                     else ghost_farfield(gas.gamma, wnew, fs, n, gw);
                 } else {
@@ -428,48 +438,44 @@ __global__ void __launch_bounds__(kWG) k_flux(DMesh m, DState s, Gas gas, double
                 for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + gcell] = gw[v];
             }
     }
-    if (LAST) {  // per-tile partial sums of the squared residual (fixed tree: deterministic)
-        __shared__ double red[kWG / 64][5];
+    if (LAST) {  // per-wavefront partial sums of the squared residual (fixed tree: deterministic)
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             double x = res2[v];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
-        }
-        __syncthreads();
-        if (threadIdx.x < 5) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
-            s.part[(size_t)tile * 8 + threadIdx.x] = t;
+            if ((threadIdx.x & 63) == 0) s.part[(size_t)(c >> 6) * 8 + v] = x;
         }
     }
 }
 
 // ------------------------------------------------------------ host launchers
 
-void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, double k3, double omega,
+void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, double k3, double omega, int wg,
                          hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    const dim3 gr(xcd_grid(nt)), bl(kWG);
-#define IL(G, V, S) hipLaunchKernelGGL((k_interp<G, V, S>), gr, bl, 0, st, m, s, k3, omega, nt)
-#define IL_S(G, V) { if (m.nslot <= 4) IL(G, V, 4); else IL(G, V, 6); }
+    const int nt = m.NRp / wg;
+    const dim3 gr(xcd_grid(nt, wg)), bl(wg);
+#define IL(G, V, S, W) hipLaunchKernelGGL((k_interp<G, V, S, W>), gr, bl, 0, st, m, s, k3, omega, nt)
+#define IL_W(G, V, S) { if (wg == 64) IL(G, V, S, 64); else if (wg == 128) IL(G, V, S, 128); else IL(G, V, S, 256); }
+#define IL_S(G, V) { if (m.nslot <= 4) IL_W(G, V, 4) else IL_W(G, V, 6) }
 #define IL_V(G) { if (venkat) IL_S(G, true) else IL_S(G, false) }
     if (grad == 0) IL_V(0) else IL_V(1)
 #undef IL_V
 #undef IL_S
+#undef IL_W
 #undef IL
 }
 
 void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, bool first,
-                        const Gas& g, double beta, double alpha, double cfl, double inv_gm1, hipStream_t st) {
-    const int nt = m.NRp / kWG;
-    const dim3 gr(xcd_grid(nt)), bl(kWG);
+                        const Gas& g, double beta, double alpha, double cfl, double inv_gm1, int wg, hipStream_t st) {
+    const int nt = m.NRp / wg;
+    const dim3 gr(xcd_grid(nt, wg)), bl(wg);
     // Stage 0 always has dissipation (beta_0 = 1 in every RK table), so FIRST implies DISS.
-#define FL(FX, O2, DI, LA, FI, S) \
-    hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, FI, S>), gr, bl, 0, st, m, s, g, beta, alpha, cfl, inv_gm1, nt)
-#define FL_S(FX, O2, DI, LA, FI) { if (m.nslot <= 4) FL(FX, O2, DI, LA, FI, 4); else FL(FX, O2, DI, LA, FI, 6); }
+#define FL(FX, O2, DI, LA, FI, S, W) \
+    hipLaunchKernelGGL((k_flux<FX, O2, DI, LA, FI, S, W>), gr, bl, 0, st, m, s, g, beta, alpha, cfl, inv_gm1, nt)
+#define FL_W(FX, O2, DI, LA, FI, S) \
+    { if (wg == 64) FL(FX, O2, DI, LA, FI, S, 64); else if (wg == 128) FL(FX, O2, DI, LA, FI, S, 128); else FL(FX, O2, DI, LA, FI, S, 256); }
+#define FL_S(FX, O2, DI, LA, FI) { if (m.nslot <= 4) FL_W(FX, O2, DI, LA, FI, 4) else FL_W(FX, O2, DI, LA, FI, 6) }
 #define FL_L(FX, O2, DI, FI) { if (last) FL_S(FX, O2, DI, true, FI) else FL_S(FX, O2, DI, false, FI) }
 #define FL_D(FX, O2) { if (first) FL_L(FX, O2, true, true) else if (diss) FL_L(FX, O2, true, false) else FL_L(FX, O2, false, false) }
 #define FL_O(FX) { if (order2) FL_D(FX, true) else FL_D(FX, false) }
@@ -478,6 +484,7 @@ void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, 
 #undef FL_D
 #undef FL_L
 #undef FL_S
+#undef FL_W
 #undef FL
 }
 

@@ -412,10 +412,10 @@ __global__ void __launch_bounds__(kWG) k_exchange(int n, const int* dst, const i
     }
 }
 
-// Residual-norm partial sums from stored conv/diss (granular path).
+// Residual-norm partial sums from stored conv/diss (granular path), one per
+// wavefront of 64 cells, as the last fused stage forms them.
 __global__ void __launch_bounds__(kWG) k_rms_partials(DMesh m, DState s, int ntiles) {
-    const int tile = blockIdx.x;
-    const int c = tile * kWG + threadIdx.x;
+    const int c = blockIdx.x * kWG + threadIdx.x;
     double res2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (m.nf[c] != 0) {
 #pragma unroll
@@ -424,20 +424,12 @@ __global__ void __launch_bounds__(kWG) k_rms_partials(DMesh m, DState s, int nti
             res2[v] = r * r;
         }
     }
-    __shared__ double red[kWG / 64][5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         double x = res2[v];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][v] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 5) {
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWG / 64; ++w) t += red[w][threadIdx.x];
-        s.part[(size_t)tile * 8 + threadIdx.x] = t;
+        if ((threadIdx.x & 63) == 0) s.part[(size_t)(c >> 6) * 8 + v] = x;
     }
 }
 
@@ -490,9 +482,17 @@ __global__ void __launch_bounds__(kWG) k_monitor_partials(int split, const int* 
     double t = wg_sum5(a, red);
     if (threadIdx.x < 5) mpart[((size_t)b * split + p) * 10 + 5 + threadIdx.x] = t;
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = p * kWG + threadIdx.x; q < blk_ntile[b]; q += split * kWG)
+    // tile q of kWG cells: its four wavefront partials in order
+    for (int q = p * kWG + threadIdx.x; q < blk_ntile[b]; q += split * kWG) {
+        const double* w = part + (size_t)(blk_tile0[b] + q) * (kWG / 64) * 8;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) r[v] += part[(size_t)(blk_tile0[b] + q) * 8 + v];
+        for (int v = 0; v < 5; ++v) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < kWG / 64; ++i) t += w[i * 8 + v];
+            r[v] += t;
+        }
+    }
     t = wg_sum5(r, red);
     if (threadIdx.x < 5) mpart[((size_t)b * split + p) * 10 + threadIdx.x] = t;
 }

@@ -258,15 +258,20 @@ int e3d_batch_history(e3d_ctx* ctx, int inst, e3d_monitor* rows, int max_rows);
  * reference, which runs one cell loop on one core). Workgroups are dealt to
  * the GPU's eight XCDs in turn; E3D_TILES_LINEAR maps workgroup i to tile i,
  * E3D_TILES_XCD_CONTIGUOUS gives each XCD one contiguous range of tiles,
- * E3D_TILES_XCD_CHUNKS deals runs of 16 consecutive tiles to the XCDs in
+ * E3D_TILES_XCD_CHUNKS deals runs of 4096 consecutive cells to the XCDs in
  * turn, E3D_TILES_AUTO (default) takes XCD_CONTIGUOUS for meshes of fewer
- * than 1024 tiles of 256 cells and XCD_CHUNKS for larger ones. Addresses
- * only: results are the same. */
+ * than 262,144 cells and XCD_CHUNKS for larger ones. Addresses only: results
+ * are the same. */
 #define E3D_TILES_AUTO (-1)
 #define E3D_TILES_LINEAR 0
 #define E3D_TILES_XCD_CONTIGUOUS 1
 #define E3D_TILES_XCD_CHUNKS 2
 int e3d_set_tile_order(e3d_ctx* ctx, int order);
+/* Cells per workgroup of the stage kernels: 64, 128 or 256 (default). No
+ * counterpart in the reference (one cell loop on one core). A thread owns one
+ * cell whatever the tile size and the residual-norm partial sums are formed
+ * per 64 cells, so results are the same bit for bit. */
+int e3d_set_tile_cells(e3d_ctx* ctx, int cells);
 
 /* Memory order of the cells and faces of the next uploaded mesh or batch.
  * E3D_ORDER_REFERENCE (default) keeps the reference's numbering
@@ -339,6 +344,13 @@ int e3d_mlp_time_step(e3d_mlp* mlp, int batch, int reps, double* ms, double* flo
 /* Average duration in ms of the named kernel over the last e3d_iterate call
  * (HIP events on the context's stream); kernel = "interp", "flux_update", ... */
 int e3d_kernel_time(e3d_ctx* ctx, const char* kernel, double* avg_ms, int* launches);
+/* Streaming HBM bandwidth of this GPU for a mix of reads and writes (no
+ * counterpart in the reference; it calibrates the stage kernels' roofline
+ * fractions): `bytes` of traffic per launch, `reads` planes read and `writes`
+ * planes written per element (1:0, 7:1, 3:1, 2:1, 1:1, 1:2 or 0:1), 16-byte
+ * nontemporal loads and stores, best of 1 and 4 elements per thread; GB/s
+ * over reps launches timed with HIP events after 3 warmup launches. */
+int e3d_stream_bandwidth(int device, long bytes, int reads, int writes, int reps, double* gbs);
 int e3d_enable_timing(e3d_ctx* ctx, int on);
 int e3d_sync(e3d_ctx* ctx);
 

@@ -208,6 +208,42 @@ def test_fused_tile_orders_match_oracle(tiles):
     ctx.close()
 
 
+@pytest.mark.parametrize("cells", [64, 128])
+@pytest.mark.parametrize("scheme", ["roe_gg_vk", "ausm_lsq_bj"])
+def test_fused_tile_sizes_match_oracle_and_each_other(cells, scheme):
+    """Tiles of 64 and 128 cells per workgroup (256 is the default of every
+    other test) give the oracle's values and, monitors included, the same
+    bits as tiles of 256 cells: 48x16x16 hexes in 8 blocks, every tile order."""
+    meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=3)
+    if scheme == "ausm_lsq_bj":
+        meta.update(flux=1, grad=1, lim=0, mach=0.7, aoa=3.0)
+    mesh = E.Mesh.bump(48, 16, 16, 0.1, 4, (2, 2, 2))
+    ref = gpu_context(mesh, product_params(meta))
+    href = ref.iterate(meta["iters"])
+    S = O.Solver([mesh.block(b) for b in range(mesh.nblocks)], oracle_params(meta))
+    S.init_freestream()
+    rh = S.iterate(meta["iters"], threads=8)
+    assert_states_equal(ref, S)
+    assert_monitors_close(href, rh)
+    for order in (E.TILES_LINEAR, E.TILES_XCD_CONTIGUOUS, E.TILES_XCD_CHUNKS):
+        ctx = E.Context(0)
+        ctx.set_params(product_params(meta))
+        ctx.upload(mesh)
+        ctx.set_tile_cells(cells)
+        ctx.set_tile_order(order)
+        ctx.init_freestream()
+        hist = ctx.iterate(meta["iters"])
+        for blk in range(mesh.nblocks):
+            for arr in (E.ARR_W, E.ARR_DISS, E.ARR_CONV):
+                assert (ctx.get(blk, arr).view(np.int64) == ref.get(blk, arr).view(np.int64)).all(), (order, blk, arr)
+        assert hist == href, order
+        ctx.close()
+    for bad in (32, 192, 512):
+        with pytest.raises(E.E3DError):
+            ref.set_tile_cells(bad)
+    ref.close()
+
+
 def test_fused_is_deterministic_and_resumable():
     """Two runs give identical bits; 3 + 3 iterations equal 6 in one call."""
     meta = dict(mach=0.675, aoa=1.0, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=50)
