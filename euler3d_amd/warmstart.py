@@ -118,9 +118,11 @@ def scheme_params(order, flux="roe", gradient="gg", limiter="vk", k=5.0, omega=1
 
 
 def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=None, use_graph=True, chunk=2000,
-                log=sys.stderr, history_every=0):
+                log=sys.stderr, history_every=0, deadline=None, keep_states=True):
     """Runs one batch; warm: per instance [5 x nall] initial states (None: freestream).
-    Returns per-instance results and the final real-cell states."""
+    deadline: wall-clock seconds after which no further chunk starts (instances
+    still running are reported as not converged). Returns per-instance results
+    and the final states (None each when keep_states is false)."""
     ctx = Context(0)
     ctx.set_params(make_params(0.5, 0.0, max_iter=max_iter, criterion=criterion, **scheme))
     insts = []
@@ -147,6 +149,10 @@ def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=N
                   f"({time.perf_counter() - t0:.1f}s)", file=log, flush=True)
         if active == 0 or n == 0:
             break
+        if deadline is not None and time.perf_counter() - t0 > deadline:
+            print(f"[warmstart]   wall-clock limit of {deadline:.0f}s: {active} instances stop unconverged",
+                  file=log, flush=True)
+            break
     wall = time.perf_counter() - t0
     out = []
     states = []
@@ -161,8 +167,8 @@ def solve_batch(meshes, conds, scheme, max_iter, criterion, warm=None, rms_ref=N
         out.append(dict(c, iterations=s["iterations"], stop_seconds=s["stop_seconds"], drop=s["last"]["drop"][0],
                         cl=s["last"]["cl"], cd=s["last"]["cd"], rms0=first[0]["rms"] if first else None,
                         drop0=first[0]["drop"][0] if first else None,
-                        converged=bool(s["last"]["drop"][0] <= criterion), history=hist_rows))
-        states.append(ctx.get(b, ARR_W))
+                        converged=bool(s["stopped"] and s["last"]["drop"][0] <= criterion), history=hist_rows))
+        states.append(ctx.get(b, ARR_W) if keep_states else None)
     ctx.close()
     return out, states, wall
 
@@ -557,6 +563,93 @@ def anatomy(n_train=128, seeds=(5000, 5004, 5008), nx=48, ny=16, nz=16, steps=20
                 exact_drop=criterion - 3.0, cases=cases, runs=rows)
 
 
+C4_MESH = (96, 32, 32)
+
+
+def run_c4(stage, n_train=128, n_heldout=64, criterion=-2.0, train_criterion=-1.5, train_iters=8000,
+           max_iter=25000, steps=20000, width=256, depth=5, boost=1, cold_json=None, deadline=None,
+           log=sys.stderr):
+    """The learned warm start on the 96x32x32x6 = 589,824-tet bump channel with
+    the batched second-order solver (Roe, Green-Gauss, Venkatakrishnan K = 5,
+    5-stage RK, CFL 1.5), in two runs that each fit one GPU session:
+
+    stage "cold": the held-out conditions (seeds 5000..) from freestream, one
+      batch, to `criterion` decades below their iteration-0 residual or
+      max_iter; per instance the iterations, device time, coefficients and
+      iteration-0 norms.
+    stage "warm": training fields of conditions 0..n_train-1 from the batched
+      solver (freestream start, to train_criterion or train_iters), the
+      surrogate (1 + boost networks of depth x width SiLU) trained on them,
+      then the held-out conditions from the surrogate's field, with the drop
+      measured from the cold runs' iteration-0 norms (read from cold_json),
+      and the comparison with the cold runs."""
+    nx, ny, nz = C4_MESH
+    scheme = scheme_params(2)
+    held_c = conditions(range(5000, 5000 + n_heldout))
+    t0 = time.perf_counter()
+    if stage == "cold":
+        hm = bump_meshes(held_c, nx, ny, nz)
+        cold, _, t_cold = solve_batch(hm, held_c, scheme, max_iter, criterion, history_every=100, deadline=deadline,
+                                      keep_states=False, log=log)
+        for m in hm:
+            m.close()
+        return dict(stage="cold", mesh=list(C4_MESH), order=2, criterion=criterion, max_iter=max_iter,
+                    n_heldout=n_heldout, batch_seconds=t_cold, heldout=cold, wall_seconds=time.perf_counter() - t0)
+    with open(cold_json) as f:
+        cold_run = json.load(f)
+    cold = cold_run["heldout"]
+    if cold_run["criterion"] != criterion or len(cold) != n_heldout:
+        raise ValueError("the cold runs were made with another criterion or number of held-out conditions")
+    X, T, res, t_gen, kept = training_set(conditions(range(n_train)), nx, ny, nz, scheme, train_iters,
+                                          train_criterion, log)
+    train_drops = [r["drop"] for r in res]
+    sur = Surrogate(X, T, 1 + boost, steps, width=width, depth=depth, log=log)
+    n_samples = len(X)
+    del X, T
+    hm = bump_meshes(held_c, nx, ny, nz)
+    t2 = time.perf_counter()
+    warm_states = [sur.state(m.block(0), c) for m, c in zip(hm, held_c)]
+    t_infer = time.perf_counter() - t2
+    warm, _, t_warm = solve_batch(hm, held_c, scheme, max_iter, criterion, warm=warm_states,
+                                  rms_ref=[r["rms0"] for r in cold], history_every=100, deadline=deadline,
+                                  keep_states=False, log=log)
+    del warm_states
+    for m in hm:
+        m.close()
+    rows = []
+    for c, w in zip(cold, warm):
+        rows.append(dict(seed=c["seed"], mach=c["mach"], aoa_deg=c["aoa_deg"], h=c["h"],
+                         cold_iterations=c["iterations"], warm_iterations=w["iterations"],
+                         cold_seconds=c["stop_seconds"], warm_seconds=w["stop_seconds"] + t_infer / len(cold),
+                         cold_converged=c["converged"], warm_converged=w["converged"], warm_initial_drop=w["drop0"],
+                         cl_cold=c["cl"], cl_warm=w["cl"], cd_cold=c["cd"], cd_warm=w["cd"],
+                         cl_rel_diff=abs(w["cl"] - c["cl"]) / max(abs(c["cl"]), 1e-300),
+                         cd_rel_diff=abs(w["cd"] - c["cd"]) / max(abs(c["cd"]), 1e-300),
+                         cold_history=c["history"], warm_history=w["history"]))
+    both = [r for r in rows if r["cold_converged"] and r["warm_converged"]]
+    summary = dict(stage="warm", mesh=list(C4_MESH), order=2, criterion=criterion, max_iter=max_iter,
+                   n_train=n_train, train_criterion=train_criterion, train_iters=train_iters,
+                   train_fields_at_criterion=sum(1 for d in train_drops if d <= train_criterion),
+                   train_drop_median=float(np.nanmedian(train_drops)), n_heldout=n_heldout, width=width, depth=depth,
+                   networks=1 + boost, samples=int(n_samples), train_steps=steps, generate_seconds=t_gen,
+                   train_seconds=sur.train_seconds, train_step_ms=sur.step_ms, train_tflops=sur.tflops,
+                   train_max_err=sur.train_max_err, train_mean_err=sur.train_mean_err,
+                   inference_seconds=t_infer, warm_batch_seconds=t_warm, cold_batch_seconds=cold_run["batch_seconds"],
+                   converged_both=len(both), cold_converged=sum(r["cold_converged"] for r in rows),
+                   warm_converged=sum(r["warm_converged"] for r in rows),
+                   iterations_ratio=(sum(r["cold_iterations"] for r in both) / max(1, sum(r["warm_iterations"] for r in both))),
+                   time_ratio=(sum(r["cold_seconds"] for r in both) / max(1e-12, sum(r["warm_seconds"] for r in both))),
+                   mean_warm_initial_drop=float(np.mean([r["warm_initial_drop"] for r in rows])),
+                   max_cl_rel_diff=max([r["cl_rel_diff"] for r in both], default=None),
+                   max_cd_rel_diff=max([r["cd_rel_diff"] for r in both], default=None),
+                   heldout=rows, wall_seconds=time.perf_counter() - t0)
+    sur.close()
+    print(f"[warmstart] C4: {len(both)}/{len(rows)} held-out reached {criterion:g} from both starts; cold/warm "
+          f"iterations {summary['iterations_ratio']:.2f}x, time {summary['time_ratio']:.2f}x; warm start's mean "
+          f"initial drop {summary['mean_warm_initial_drop']:.2f}", file=log, flush=True)
+    return summary
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=128)
@@ -585,8 +678,23 @@ def main(argv=None):
                     help="the 37.7M-cell AUSM/LSQ/BJ channel (384x128x128, M=0.7, 3 deg, h=0.1) instead")
     ap.add_argument("--large-iters", type=int, default=3000)
     ap.add_argument("--train-iters", type=int, default=8000)
+    ap.add_argument("--c4", choices=["cold", "warm"], default=None,
+                    help="the 96x32x32x6 channel at second order: held-out cold runs, or training + warm runs "
+                         "(needs --cold-json from the cold stage)")
+    ap.add_argument("--cold-json", default=None)
+    ap.add_argument("--deadline", type=float, default=None,
+                    help="wall-clock seconds after which a held-out batch stops (unfinished instances reported)")
     ap.add_argument("--json")
     a = ap.parse_args(argv)
+    if a.c4:
+        s = run_c4(a.c4, n_train=a.train, n_heldout=a.heldout, criterion=a.criterion,
+                   train_criterion=-1.5 if a.train_criterion is None else a.train_criterion,
+                   train_iters=a.train_iters, max_iter=a.max_iter, steps=a.steps, width=a.width,
+                   boost=a.boost, cold_json=a.cold_json, deadline=a.deadline)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(s, f, indent=1)
+        return s
     if a.anatomy:
         s = anatomy(a.train, nx=a.nx, ny=a.nx // 3, nz=a.nx // 3, steps=a.steps, width=a.width, max_iter=a.max_iter,
                     criterion=a.criterion)

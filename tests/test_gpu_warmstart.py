@@ -52,3 +52,30 @@ def test_error_memory_keeps_the_answer_the_methods():
         assert r["warm_iterations"] < r["cold_iterations"], info
         assert r["deep_converged"], info
         assert r["cl_deep_rel_diff"] <= 1e-7 and r["cd_deep_rel_diff"] <= 1e-7, info
+
+
+def test_c4_pipeline_on_the_full_mesh(tmp_path):
+    """The C4 pipeline (run_c4) at full mesh size, 96x32x32x6 = 589,824 tets,
+    second-order Roe / Green-Gauss / Venkatakrishnan, with a few fields: two
+    held-out conditions run cold to a shallow criterion (-0.7 decades), four
+    training fields from the batched solver, two 5 x 256 networks, then the
+    warm runs measured from the cold runs' iteration-0 norms. The surrogate's
+    field must start below the cold residual, stay finite and stop sooner."""
+    import json
+    print("c4 pipeline: 2 held-out cold on 96x32x32x6", flush=True)
+    cold = ws.run_c4("cold", n_heldout=2, criterion=-0.7, max_iter=3000)
+    assert all(r["converged"] for r in cold["heldout"]), [r["drop"] for r in cold["heldout"]]
+    path = str(tmp_path / "cold.json")
+    with open(path, "w") as f:
+        json.dump(cold, f)
+    print("c4 pipeline: 4 training fields, 2 networks, 2 held-out warm", flush=True)
+    s = ws.run_c4("warm", n_train=4, n_heldout=2, criterion=-0.7, train_criterion=-0.7, train_iters=600,
+                  max_iter=3000, steps=1500, width=256, boost=1, cold_json=path)
+    assert s["mesh"] == [96, 32, 32] and s["samples"] == 4 * 589824
+    assert s["converged_both"] == 2
+    for r in s["heldout"]:
+        info = {k: r[k] for k in ("seed", "cold_iterations", "warm_iterations", "warm_initial_drop")}
+        print(info, flush=True)
+        assert r["warm_initial_drop"] < 0.0, info
+        assert r["warm_iterations"] < r["cold_iterations"], info
+        assert np.isfinite([r["cl_warm"], r["cd_warm"]]).all(), info
