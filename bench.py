@@ -57,7 +57,34 @@ SCHEMES = {
                         label="2nd-order AUSM + least squares + Barth-Jespersen omega=1e-16, M=0.7, 3 deg"),
     "roe-1st": dict(mach=0.5, aoa=0.0, flux=0, gradient=0, limiter=1, k=5.0, omega=1e-16, order=1,
                     label="1st-order Roe, M=0.5"),
+    # the reference's own ONERA M6 input (input_onera120)
+    "roe-lsq-vk": dict(mach=0.8, aoa=1.25, flux=0, gradient=1, limiter=1, k=5.0, omega=2.16e-16,
+                       label="2nd-order Roe + least squares + Venkatakrishnan K=5, M=0.8, 1.25 deg"),
 }
+ONERA_FIXTURE = os.path.join(ROOT, "tests", "golden", "onera_m6_120b.npz")
+
+
+def onera_mesh(E):
+    """The reference's ONERA M6 mesh in 120 blocks (582,752 tets, its own
+    partitioner's output), written from the byte-for-byte copy kept with the
+    tests into a scratch directory and read by this build's reader."""
+    import tempfile
+
+    import numpy as np
+    d = tempfile.mkdtemp(prefix="e3d_onera_")
+    z = np.load(ONERA_FIXTURE, allow_pickle=False)
+    mdir = os.path.join(d, "Meshes", "oneraM6_120B")
+    os.makedirs(mdir)
+    for k in z.files:
+        if k.startswith("file:"):
+            with open(os.path.join(mdir, k[5:]), "wb") as f:
+                f.write(z[k].tobytes())
+    cwd = os.getcwd()
+    os.chdir(d)  # the topology names its block files relative to the reference tree's root
+    try:
+        return E.Mesh.read(os.path.join(mdir, "topology_M6.top"))
+    finally:
+        os.chdir(cwd)
 
 
 def algorithmic_bytes(blocks, lsq=False, order=2):
@@ -225,6 +252,54 @@ def stream_rate(table, write_fraction):
     return pts[-1][1]
 
 
+# Warm-start leg of the bench line: the learned warm start against the cold
+# (freestream) start on a configuration that fits the bench's time budget.
+WARM_START = dict(nx=48, n_train=256, train_criterion=-6.0, n_heldout=16, criterion=-8.0, max_iter=30000,
+                  steps=20000, width=512, boost=2, deeper=3.0)
+
+
+def warm_start_leg():
+    """Time to an 8-decade residual drop from the surrogate's field vs from
+    freestream: training fields from the batched solver, the networks trained
+    on the GPU in the run, held-out conditions run cold and warm (drop measured
+    from the cold run's iteration-0 residual), both then converged three decades
+    further to compare Cl and Cd."""
+    from euler3d_amd import warmstart as ws
+    c = WARM_START
+    t0 = time.perf_counter()
+    s = ws.run(n_train=c["n_train"], n_heldout=c["n_heldout"], nx=c["nx"], ny=c["nx"] // 3, nz=c["nx"] // 3, order=1,
+               max_iter=c["max_iter"], criterion=c["criterion"], steps=c["steps"], width=c["width"],
+               deeper=c["deeper"], train_criterion=c["train_criterion"], boost=c["boost"])
+    both = [r for r in s["heldout"] if r["cold_converged"] and r["warm_converged"]]
+    out = {
+        "config": f"bump channel {c['nx']}x{c['nx'] // 3}x{c['nx'] // 3}x6 tets, 1st-order Roe, 5-stage RK, CFL 1.5; "
+                  f"{c['n_train']} training fields (seeds 0-{c['n_train'] - 1}) to a {-c['train_criterion']:g}-decade "
+                  f"drop; {1 + c['boost']} networks 25 -> 5x{c['width']} SiLU -> 5 (BF16 MFMA, FP32 accumulation), "
+                  f"{c['steps']} Adam steps each; {c['n_heldout']} held-out conditions (seeds 5000-"
+                  f"{5000 + c['n_heldout'] - 1})",
+        "drop": c["criterion"], "drop_reference": "cold start's iteration-0 residual of the same condition",
+        "converged_both": len(both), "heldout": c["n_heldout"],
+        "cold_iterations": sum(r["cold_iterations"] for r in both),
+        "warm_iterations": sum(r["warm_iterations"] for r in both),
+        "iterations_ratio": s["iterations_ratio"],
+        "cold_seconds": sum(r["cold_seconds"] for r in both),
+        "warm_seconds": sum(r["warm_seconds"] for r in both),
+        "time_ratio": s["time_ratio"],
+        "time_includes": "device time of each instance in its batch; warm adds its share of the surrogate's inference",
+        "iterations_ratio_own_r0": s["iterations_ratio_own_r0"],
+        "max_cl_rel_diff_at_drop": s["max_cl_rel_diff"], "max_cd_rel_diff_at_drop": s["max_cd_rel_diff"],
+        "max_cl_rel_diff_converged": s["max_cl_deep_rel_diff"], "max_cd_rel_diff_converged": s["max_cd_deep_rel_diff"],
+        "converged_drop": c["criterion"] - c["deeper"],
+        "train_seconds": s["train_seconds"], "generate_seconds": s["generate_seconds"],
+        "train_tflops": s["train_tflops"], "wall_seconds": time.perf_counter() - t0,
+    }
+    print(f"[bench] warm start: {out['converged_both']}/{c['n_heldout']} held-out reached {c['criterion']:g}; "
+          f"cold/warm iterations {out['iterations_ratio']:.2f}x, time {out['time_ratio']:.2f}x; Cl/Cd rel diff "
+          f"{out['max_cl_rel_diff_converged']}/{out['max_cd_rel_diff_converged']} at {out['converged_drop']:g}; "
+          f"{out['wall_seconds']:.0f}s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +310,9 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--parts", type=int, nargs=3, default=[2, 2, 2])
     ap.add_argument("--scheme", choices=sorted(SCHEMES), default="roe-gg-vk")
+    ap.add_argument("--mesh", choices=["bump", "onera"], default="bump",
+                    help="onera: the reference's ONERA M6 mesh in 120 blocks with its own input's scheme "
+                         "(implies --scheme roe-lsq-vk)")
     ap.add_argument("--brick", type=int, default=4,
                     help="cell numbering of the generated mesh: bricks of brick^3 hexes (0 lexicographic), "
                          "or < 0 for tets in a pseudo-random order seeded by -brick")
@@ -247,6 +325,8 @@ def main():
                     help="cells per workgroup of the stage kernels (results are the same bit for bit)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the warm-start leg (time to an 8-decade drop, learned warm start vs cold start)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy: the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -263,10 +343,16 @@ def main():
 
     import euler3d_amd as E
 
+    if a.mesh == "onera":
+        a.scheme = "roe-lsq-vk"
     sc = SCHEMES[a.scheme]
-    print(f"[bench] building the {a.nx}x{a.ny}x{a.nz}x6 mesh in {a.parts} blocks", file=sys.stderr, flush=True)
     t_mesh = time.perf_counter()
-    mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, a.brick, tuple(a.parts))
+    if a.mesh == "onera":
+        print("[bench] reading the ONERA M6 mesh (120 blocks)", file=sys.stderr, flush=True)
+        mesh = onera_mesh(E)
+    else:
+        print(f"[bench] building the {a.nx}x{a.ny}x{a.nz}x6 mesh in {a.parts} blocks", file=sys.stderr, flush=True)
+        mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, a.brick, tuple(a.parts))
     blocks = [mesh.block(b) for b in range(mesh.nblocks)]
     t_mesh = time.perf_counter() - t_mesh
     order = sc.get("order", 2)
@@ -346,16 +432,19 @@ def main():
     if not a.no_cpu_baseline:
         cpu, cpu1 = cpu_baselines(a, sc, blocks)
         cpu_ref = reference_baseline(sc)
+    warm = None if a.no_warm_start else warm_start_leg()
 
     out = {
         "metric": "Mcell-RK-stage/s (FP64, %HBM roof) + time to 8-order residual drop, warm vs cold",
         "value": value, "unit": "Mcell-RK-stage/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"bump channel {a.nx}x{a.ny}x{a.nz}x6 tets ({N} cells), "
-                               f"{mesh.nblocks} block(s) on one device, {sc['label']}, 5-stage RK, CFL 1.5",
+        "config": {"workload": (f"bump channel {a.nx}x{a.ny}x{a.nz}x6 tets ({N} cells)" if a.mesh == "bump" else
+                                f"the reference's ONERA M6 mesh ({N} tets, Meshes/oneraM6_120B)")
+                               + f", {mesh.nblocks} block(s) on one device, {sc['label']}, 5-stage RK, CFL 1.5",
                    "cells": N, "ghosts": G, "faces": F, "blocks": mesh.nblocks, "stages": stages,
-                   "cell_numbering": "scrambled" if a.brick < 0 else f"brick {a.brick}",
+                   "cell_numbering": ("the mesh file's" if a.mesh == "onera" else
+                                      "scrambled" if a.brick < 0 else f"brick {a.brick}"),
                    "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate",
                    "tile_cells": a.tile_cells,
                    "tile_order": "automatic by mesh size (XCD-contiguous below 262,144 cells, "
@@ -368,6 +457,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_1core": cpu1,
         "cpu_baseline_reference": cpu_ref,
+        "warm_start": warm,
     }
     print(json.dumps(out))
 

@@ -10,8 +10,11 @@ residual has dropped ``criterion`` decades (its own stop test) or it reaches
 ``Context.batch_run``: every RK stage is one launch over all instances, the
 iteration replayed as a hipGraph, converged instances skipped).
 
-Cell ordering (brick size) per size class and the tile order are chosen by the
-autotuner (``euler3d_amd.autotune``).
+Every mesh is numbered by the generator in bricks of 4 hexahedra (fixed: the
+numbering sets the summation order, so the results do not depend on any
+timing). The device memory order, the tile order and the tile shape (cells
+per workgroup), which move addresses only, are chosen by the autotuner
+(``euler3d_amd.autotune``) per size class and then by a vote weighted by cells.
 
 usage: python -m euler3d_amd.sweep [--instances 64] [--max-iter 1000] [--criterion -8]
                                    [--scale 1.0] [--no-graph] [--no-autotune] [--json out.json]
@@ -49,35 +52,40 @@ def instances(n=64, seed0=1000, scale=1.0):
 def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune=True, seed0=1000, log=None):
     from . import autotune as at
     cases = instances(n, seed0, scale)
-    tile_votes = {0: 0.0, 1: 0.0, 2: 0.0}
-    mem_votes = {}
+    tile_votes, mem_votes, cell_votes = {}, {}, {}
     t0 = time.perf_counter()
     for c in cases:
         sc = at.size_class(c["nx"])
-        brick, order, mem = at.tune(sc, sc // 3, sc // 3, SCHEME, verbose=bool(log)) if autotune else (4, 1, 0)
-        c["brick"] = brick
+        order, mem, cells = at.tune(sc, sc // 3, sc // 3, SCHEME, verbose=bool(log)) if autotune else (-1, 0, 256)
+        c["brick"] = at.DEFAULT_BRICK
         c["cells"] = 6 * c["nx"] * c["ny"] * c["nz"]
-        tile_votes[order] += c["cells"]
+        c["tuned"] = dict(tile_order=order, memory_order=mem, tile_cells=cells)
+        tile_votes[order] = tile_votes.get(order, 0.0) + c["cells"]
         mem_votes[mem] = mem_votes.get(mem, 0.0) + c["cells"]
+        cell_votes[cells] = cell_votes.get(cells, 0.0) + c["cells"]
     t_tune = time.perf_counter() - t0
     # the mesh builder runs outside the interpreter lock: build the instances in parallel
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=8) as pool:
         meshes = list(pool.map(lambda c: Mesh.bump(c["nx"], c["ny"], c["nz"], c["h"], c["brick"], (1, 1, 1)), cases))
     t_mesh = time.perf_counter() - t0 - t_tune
+    # one device memory order, tile order and tile shape for the whole batch, by cells
     tile_order = max(tile_votes, key=tile_votes.get)
-    memory_order = max(mem_votes, key=mem_votes.get)  # one device memory order for the whole batch, by cells
+    memory_order = max(mem_votes, key=mem_votes.get)
+    tile_cells = max(cell_votes, key=cell_votes.get)
     ctx = Context(0)
     ctx.set_params(make_params(0.5, 0.0, max_iter=max_iter, criterion=criterion, **SCHEME))
     ctx.set_ordering(memory_order)
     ctx.upload_batch(meshes, [dict(mach=c["mach"], aoa_deg=c["aoa_deg"], cmac=1.0, criterion=criterion,
                                    max_iter=max_iter) for c in cases])
     ctx.set_tile_order(tile_order)
+    ctx.set_tile_cells(tile_cells)
     ctx.init_freestream()
     ctx.op("sync")
     if log:
         print(f"[sweep] {n} instances, {sum(c['cells'] for c in cases)} cells, tuned in {t_tune:.1f}s, meshes built "
-              f"in {t_mesh:.1f}s, tile order {tile_order}, memory order {memory_order}", file=log, flush=True)
+              f"in {t_mesh:.1f}s, tile order {tile_order}, memory order {memory_order}, tile {tile_cells} cells",
+              file=log, flush=True)
     t1 = time.perf_counter()
     done, status = ctx.batch_run(max_iter, use_graph=use_graph)
     wall = time.perf_counter() - t1
@@ -96,6 +104,7 @@ def run(n=64, max_iter=1000, criterion=-8.0, scale=1.0, use_graph=True, autotune
             "cells": sum(c["cells"] for c in cases), "mcell_rk_stage_per_s": cell_stages / wall / 1e6,
             "reached_criterion": sum(c["reached_criterion"] for c in cases),
             "diverged": [c["seed"] for c in cases if c["diverged"]], "tile_order": tile_order,
+            "memory_order": memory_order, "tile_cells": tile_cells,
             "mesh_seconds": t_mesh, "tune_seconds": t_tune, "max_iter": max_iter, "criterion": criterion, "graph": use_graph}
 
 

@@ -13,7 +13,8 @@ import euler3d_amd as E
 import oracle as O
 
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))
+                  if not os.path.basename(p).startswith("onera_"))  # a mesh, not a golden run
 SMALL_FIXTURES = [f for f in FIXTURES if not f.startswith("c1_")]
 C1_CASE = "c1_roe1"  # 48x16x16x6 tets, first-order Roe, M=0.5, 200 iterations
 
@@ -190,3 +191,35 @@ def write_case_mesh(g, directory):
     top = m.write(directory, "case")
     m.close()
     return top
+
+
+ONERA_FIXTURE = os.path.join(GOLDEN_DIR, "onera_m6_120b.npz")
+
+
+def onera_files(directory):
+    """Writes the reference's ONERA M6 mesh (topology + 120 SU2 block files, as
+    the reference ships them) and its input file under directory, laid out as
+    in the reference tree (the topology names its block files relative to the
+    tree's root); returns (topology path, input path)."""
+    z = np.load(ONERA_FIXTURE, allow_pickle=False)
+    mdir = os.path.join(directory, "Meshes", "oneraM6_120B")
+    os.makedirs(mdir, exist_ok=True)
+    for k in z.files:
+        if k.startswith("file:"):
+            with open(os.path.join(mdir, k[5:]), "wb") as f:
+                f.write(z[k].tobytes())
+    inp = os.path.join(directory, "input_onera120")
+    with open(inp, "wb") as f:
+        f.write(z["input"].tobytes())
+    return os.path.join(mdir, "topology_M6.top"), inp
+
+
+def onera_mesh(directory):
+    """The ONERA M6 mesh read by this build's reader from the fixture's files."""
+    top, inp = onera_files(directory)
+    cwd = os.getcwd()
+    os.chdir(directory)
+    try:
+        return E.Mesh.read(top), inp
+    finally:
+        os.chdir(cwd)

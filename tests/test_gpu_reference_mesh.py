@@ -1,0 +1,66 @@
+"""The reference's own unstructured mesh on the GPU: ONERA M6 in 120 blocks
+(Meshes/oneraM6_120B, 582,752 tets written by the reference's partitioner,
+read from the byte-for-byte copy in tests/golden/onera_m6_120b.npz) with the
+reference's own input file (input_onera120: 2nd-order Roe, least squares,
+Venkatakrishnan K = 5, M = 0.8 at 1.25 deg).
+
+The reference's serial build stops with a segmentation fault on these files
+(its reader runs past the markers a block file lists), so no reference run
+exists to compare with: parity here is against the oracle (the restatement
+pinned bit for bit on the golden runs), two fused iterations in the file's
+cell order and in the Morton order, every state and dissipation value equal,
+monitors within 1e-12."""
+import os
+
+import numpy as np
+import pytest
+
+import euler3d_amd as E
+import oracle as O
+from helpers import onera_mesh, values_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_close(a, b, rtol=1e-12):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return bool(np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.fixture(scope="module")
+def onera(tmp_path_factory):
+    mesh, inp = onera_mesh(str(tmp_path_factory.mktemp("onera")))
+    p, _ = E.read_input(inp)
+    p.max_iter, p.criterion = 10, -15.0
+    yield mesh, p
+    mesh.close()
+
+
+def test_onera_m6_fused_iterations_match_oracle(onera):
+    mesh, p = onera
+    assert mesh.nblocks == 120
+    print("onera m6: oracle, 2 iterations on 16 threads", flush=True)
+    S = O.Solver([mesh.block(b) for b in range(mesh.nblocks)],
+                 O.params(p.mach, p.aoa_deg, flux=p.flux, order=p.order, gradient=p.gradient, limiter=p.limiter,
+                          omega=p.omega, k=p.k, max_iter=10, criterion=-15.0))
+    S.init_freestream()
+    rh = S.iterate(2, threads=min(16, os.cpu_count() or 1))
+    for order in (E.ORDER_REFERENCE, E.ORDER_MORTON):
+        print(f"onera m6: device, memory order {order}", flush=True)
+        ctx = E.Context(0)
+        ctx.set_params(p)
+        ctx.set_ordering(order)
+        ctx.upload(mesh)
+        ctx.init_freestream()
+        hist = ctx.iterate(2)
+        for b, B in enumerate(S.blocks):
+            W = ctx.get(b, E.ARR_W).reshape(-1)
+            same = values_equal(W, B.W)
+            assert same.all(), f"order {order} block {b}: {np.sum(~same)} state values differ"
+            d = ctx.get(b, E.ARR_DISS).reshape(-1)
+            same = values_equal(d, B.diss.reshape(5, B.nall)[:, :B.nreal].reshape(-1))
+            assert same.all(), f"order {order} block {b}: dissipation differs"
+        for a, r in zip(hist, rh):
+            assert _rel_close(a["rms"], r["rms"]), (order, a["iter"], a["rms"], r["rms"])
+            assert _rel_close([a["cl"], a["cd"]], [r["cl"], r["cd"]]), (order, a["iter"])
+        ctx.close()
