@@ -374,6 +374,10 @@ def main():
     t_up = time.perf_counter() - t_up
     print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s, uploaded "
           f"in {t_up:.1f}s); warmup {a.warmup}", file=sys.stderr, flush=True)
+    if a.graph and a.warmup % 2:
+        # the graph holds two iterations (an even number of state-buffer swaps): an odd
+        # warmup would leave the buffers swapped and re-capture the graph inside the timed region
+        a.warmup += 1
     if a.warmup:
         if a.graph:
             ctx.batch_run(a.warmup, use_graph=True)

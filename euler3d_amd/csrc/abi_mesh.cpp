@@ -51,7 +51,10 @@ void build_blocks(e3d_mesh& m) {
     };
     const size_t nt = std::min<size_t>(nb, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     std::vector<std::thread> pool;
-    for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    try {
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    } catch (...) {  // no more threads: the ones started and this one take the remaining blocks
+    }
     work();
     for (auto& t : pool) t.join();
     for (auto& e : err)

@@ -213,7 +213,8 @@ class Surrogate:
     def __init__(self, X, T, stages=1, steps=30000, batch=8192, lr=2e-3, width=256, depth=5, log=sys.stderr,
                  var_weights=None):
         """var_weights: relative loss weight of (rho, u, v, w, p), 1 each by default; a variable's
-        targets are divided by its weight, so its squared error counts weight^2 times."""
+        targets are multiplied by its weight for training and its predictions divided by it,
+        so its squared error counts weight^2 times."""
         self.nets, self.scales, self.loss = [], [], None
         t1 = time.perf_counter()
         wv = np.ones(5, np.float32) if var_weights is None else np.asarray(var_weights, np.float32)
