@@ -246,6 +246,8 @@ def stream_rate(table, write_fraction):
     """Streaming rate of a traffic mix: linear in the write fraction between the
     two measured mixes around it."""
     pts = sorted((m["write_fraction"], m["GB/s"]) for m in table)
+    if write_fraction <= pts[0][0]:
+        return pts[0][1]
     for (x0, y0), (x1, y1) in zip(pts, pts[1:]):
         if x0 <= write_fraction <= x1:
             return y0 + (y1 - y0) * (write_fraction - x0) / (x1 - x0)
@@ -415,6 +417,27 @@ def main():
                            "frac": ach / HBM_PEAK_GBS, "traffic": measured.get(name),
                            "algorithmic_bytes_per_launch": alg[name], "avg_ms": t["avg_ms"],
                            "share_of_step": t["avg_ms"] * t["launches"] / (wall * 1e3)}
+    # the kernels' physical HBM rate against what the library's streaming kernel
+    # reaches on this GPU for the same read/write mix (measured live, 2 GiB per launch)
+    tf = None
+    if tfile and os.path.exists(os.path.join(ROOT, "profiles", tfile)):
+        with open(os.path.join(ROOT, "profiles", tfile)) as f:
+            tf = json.load(f)
+    if tf and rooflines:
+        import ctypes
+        cal = tf["calibration"]
+        stream = []
+        for r_, w_ in ((7, 1), (3, 1), (2, 1), (1, 1)):
+            g = ctypes.c_double()
+            E._check(E.lib().e3d_stream_bandwidth(0, 2 << 30, r_, w_, 5, ctypes.byref(g)))
+            stream.append({"write_fraction": w_ / (r_ + w_), "GB/s": g.value})
+        for name, rl in rooflines.items():
+            rd = tf["fetch_size_kib"][name] * 1024 * cal["fetch_factor"]
+            wr = tf["write_size_kib"][name] * 1024 * cal["write_factor"]
+            rate = (rd + wr) / (rl["avg_ms"] * 1e-3) / 1e9
+            ceil = stream_rate(stream, wr / (rd + wr))
+            rl.update(hbm_rate_gbs=rate, write_fraction=wr / (rd + wr), stream_ceiling_gbs=ceil,
+                      frac_of_stream_ceiling=rate / ceil)
     dominant = max(rooflines, key=lambda k: rooflines[k]["share_of_step"]) if rooflines else None
     rms_ro = hist[-1]["rms"][0] if hist else float("nan")
     print(f"[bench] {a.steps} steps in {wall:.3f}s; rms_ro {rms_ro:.6e}; kernels "
