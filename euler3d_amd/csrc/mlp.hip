@@ -140,56 +140,94 @@ __global__ void __launch_bounds__(kThreads) k_gemm(int M, int N, int ksplit, con
 // The same product with the workgroup's operand tiles staged in LDS: a
 // 128 x 128 output tile per workgroup, 64 x 64 (4 x 4 MFMA tiles) per wave,
 // so each 16-byte operand fetched from memory feeds 16 MFMAs of the
-// workgroup instead of 2-4. The next k-step's tiles are loaded into registers
-// while this step's MFMAs run. Every output element accumulates over k in
-// the same order as k_gemm, so the two give the same bits. M and N must be
-// multiples of 128.
-template <int EPI>
+// workgroup instead of 2-4. A k-step covers KS = 32 or 64 columns of k; the
+// next k-step's tiles are loaded into registers while this step's MFMAs run
+// and then stored into the second of two LDS buffers, so a k-step needs one
+// barrier. Every output element accumulates over k in 32-wide MFMAs in
+// ascending order, as in k_gemm, so the two give the same bits for any KS.
+// M and N must be multiples of 128, the k range of a split a multiple of KS.
+template <int EPI, int KS>
 __global__ void __launch_bounds__(kThreads) k_gemm_lds(int M, int N, int ksplit, const bf16_t* __restrict__ A,
                                                        int lda, const bf16_t* __restrict__ B, int ldb, EpiArgs e) {
-    constexpr int T = 128, KS = 32, LD = KS + 8;  // tile, k-step, LDS row stride in bf16 (80 bytes)
-    __shared__ bf16_t As[T * LD], Bs[T * LD];
+    constexpr int T = 128, LD = KS + 8;   // tile; LDS row stride in bf16 (a bank-conflict-free ds_read_b128 pattern)
+    constexpr int CPR = KS / 8;           // 16-byte chunks per tile row and k-step
+    constexpr int NCH = T * CPR / kThreads;  // chunks per thread and operand: 2 or 4
+    __shared__ __align__(16) bf16_t smem[4 * T * LD];  // A buffers 0, 1, then B buffers 0, 1
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int mt = blockIdx.x * T, nt = blockIdx.y * T;
     const int k0 = blockIdx.z * ksplit, k1 = k0 + ksplit;
     const int r = lane & 15, g = lane >> 4;
-    // this thread's two 16-byte chunks of each operand tile: rows tid / 4 and
-    // 64 + tid / 4, columns 8 (tid % 4) .. + 7 of the k-step
-    const int row0 = tid >> 2, row1 = row0 + 64, part = (tid & 3) * 8;
+    // this thread's chunks of each operand tile: chunk c is row (tid + 256 c) / CPR,
+    // columns 8 ((tid + 256 c) % CPR) .. + 7 of the k-step
+    const int part = (tid % CPR) * 8;
+    int lofs[NCH];
+    const bf16_t* ga_p[NCH];
+    const bf16_t* gb_p[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int row = (tid + kThreads * c) / CPR;
+        lofs[c] = row * LD + part;
+        ga_p[c] = A + (size_t)(mt + row) * lda + part;
+        gb_p[c] = B + (size_t)(nt + row) * ldb + part;
+    }
     floatx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const bf16_t* a0 = A + (size_t)(mt + row0) * lda + part;
-    const bf16_t* a1 = A + (size_t)(mt + row1) * lda + part;
-    const bf16_t* b0 = B + (size_t)(nt + row0) * ldb + part;
-    const bf16_t* b1 = B + (size_t)(nt + row1) * ldb + part;
-    bf16x8 ga0 = *reinterpret_cast<const bf16x8*>(a0 + k0), ga1 = *reinterpret_cast<const bf16x8*>(a1 + k0);
-    bf16x8 gb0 = *reinterpret_cast<const bf16x8*>(b0 + k0), gb1 = *reinterpret_cast<const bf16x8*>(b1 + k0);
+    bf16x8 ga[NCH], gb[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k0);
+        gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k0);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        *reinterpret_cast<bf16x8*>(smem + lofs[c]) = ga[c];
+        *reinterpret_cast<bf16x8*>(smem + 2 * T * LD + lofs[c]) = gb[c];
+    }
+    __syncthreads();
+    int cur = 0;
     for (int k = k0; k < k1; k += KS) {
-        __syncthreads();  // the previous step's LDS reads are done
-        *reinterpret_cast<bf16x8*>(As + row0 * LD + part) = ga0;
-        *reinterpret_cast<bf16x8*>(As + row1 * LD + part) = ga1;
-        *reinterpret_cast<bf16x8*>(Bs + row0 * LD + part) = gb0;
-        *reinterpret_cast<bf16x8*>(Bs + row1 * LD + part) = gb1;
-        __syncthreads();
-        if (k + KS < k1) {  // uniform: prefetch the next k-step
-            ga0 = *reinterpret_cast<const bf16x8*>(a0 + k + KS);
-            ga1 = *reinterpret_cast<const bf16x8*>(a1 + k + KS);
-            gb0 = *reinterpret_cast<const bf16x8*>(b0 + k + KS);
-            gb1 = *reinterpret_cast<const bf16x8*>(b1 + k + KS);
+        const bool more = k + KS < k1;  // uniform
+        if (more) {  // the next k-step's tiles, in flight while this step's MFMAs run
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k + KS);
+                gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k + KS);
+            }
         }
-        bf16x8 a[4], b[4];
+        const bf16_t* Ac = smem + cur * T * LD;
+        const bf16_t* Bc = smem + (2 + cur) * T * LD;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(As + (wm * 64 + 16 * i + r) * LD + 8 * g);
+        for (int kk = 0; kk < KS; kk += 32) {
+            bf16x8 a[4], b[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn * 64 + 16 * j + r) * LD + 8 * g);
+            for (int i = 0; i < 4; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(Ac + (wm * 64 + 16 * i + r) * LD + kk + 8 * g);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(Bc + (wn * 64 + 16 * j + r) * LD + kk + 8 * g);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {  // into the other buffer, which no wave reads during this step
+            bf16_t* An = smem + (cur ^ 1) * T * LD;
+            bf16_t* Bn = smem + (2 + (cur ^ 1)) * T * LD;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                *reinterpret_cast<bf16x8*>(An + lofs[c]) = ga[c];
+                *reinterpret_cast<bf16x8*>(Bn + lofs[c]) = gb[c];
+            }
+        }
+        // this step's reads of buffer cur and writes of the other are done before
+        // the next step reads the other and (one step later) overwrites cur
+        __syncthreads();
+        cur ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -442,8 +480,14 @@ void gemm(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda,
 template <int EPI>
 void gemm_n(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda, const bf16_t* B, int ldb,
             const EpiArgs& e) {
-    if (M % 128 == 0 && N % 128 == 0)
-        hipLaunchKernelGGL((k_gemm_lds<EPI>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
+    // a 64-wide k-step (half the barriers) pays off for the weight-gradient
+    // products, whose plain-store epilogue leaves the main loop as the cost;
+    // the SiLU-epilogue products measured faster with 32
+    if (EPI == EPI_STORE && M % 128 == 0 && N % 128 == 0 && (K / splits) % 64 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 64>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
+                           K / splits, A, lda, B, ldb, e);
+    else if (M % 128 == 0 && N % 128 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 32>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
                            K / splits, A, lda, B, ldb, e);
     else if (N % 64 == 0) gemm<EPI, 4>(m, M, N, K, splits, A, lda, B, ldb, e);
     else gemm<EPI, 2>(m, M, N, K, splits, A, lda, B, ldb, e);
