@@ -204,13 +204,14 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
     }
 }
 
-template <int GRAD, bool VENKAT, int MAXS, int WG>
-__global__ void __launch_bounds__(WG) k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
-    const int tile = tile_of<WG>(m, ntiles);
-    if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
-    const int c = tile * WG + threadIdx.x;
-    const int nf = ld32_stream(m.nf, c);
-    if (nf == 0) return;
+// One cell of k_interp. FULL: every cell of the wavefront has MAXS faces (a
+// tet mesh with MAXS = 4 always does), so the per-slot tests drop out and the
+// face slots are straight-line code whose loads the compiler can overlap;
+// otherwise slot k is skipped where k >= nf. Both forms evaluate the same
+// expressions in the same order.
+template <int GRAD, bool VENKAT, int MAXS, int WG, bool FULL>
+__device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, double k3, double omega, int c, int nf,
+                                            double (*r_lds)[WG]) {
     const int NRp = m.NRp, ns = m.nslot;
     double wc[5];
     load_w32(s.W, m.NA, c, wc);
@@ -219,7 +220,6 @@ __global__ void __launch_bounds__(WG) k_interp(DMesh m, DState s, double k3, dou
     // The neighbour ids of all slots are loaded first (slots k >= nf hold
     // in-bounds padding and are not used), so each face then waits for one
     // memory round trip (its neighbour's state and its geometry), not two.
-    __shared__ double r_lds[MAXS * 3][WG];  // r vectors of the cell's faces, for passes 2 and 3
     double g[15], mx[5], mn[5];
     unsigned ghost_slots = 0;  // bit k: the neighbour across face slot k is a ghost
     int nbv[MAXS];
@@ -229,40 +229,66 @@ __global__ void __launch_bounds__(WG) k_interp(DMesh m, DState s, double k3, dou
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
 #pragma unroll
     for (int v = 0; v < 5; ++v) mx[v] = mn[v] = wc[v];
+    struct Slot {
+        double wn[5], geo[3], r[3];
+    };
+    // the neighbour's state, S oriented away from this cell (Green-Gauss) or the
+    // least-squares weights, and the r vector of face slot k
+    auto load_slot = [&](int k, Slot& q) {
+        load_w32(s.W, m.NA, nbv[k], q.wn);
+        if (GRAD == 0) {
+            slot_S(m, k, c, q.geo);
+        } else {
 #pragma unroll
-    for (int k = 0; k < MAXS; ++k)
-        if (k < nf) {
-            double wn[5], geo[3], r[3];
-            load_w32(s.W, m.NA, nbv[k], wn);
-            // S oriented away from this cell (Green-Gauss) or the least-squares
-            // weights; the r vector goes to LDS for the limiter and reconstruction
-            if (GRAD == 0) {
-                slot_S(m, k, c, geo);
-            } else {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) geo[d] = ld32_stream(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
-            }
-            slot_r(m, k, c, r);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) r_lds[k * 3 + d][threadIdx.x] = r[d];
-            if (nbv[k] >= NRp) ghost_slots |= 1u << k;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                if (wn[v] > mx[v]) mx[v] = wn[v];
-                if (wn[v] < mn[v]) mn[v] = wn[v];
-            }
-            if (GRAD == 0) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) g[v * 3 + d] += 0.5 * (wc[v] + wn[v]) * geo[d];
-            } else {
-#pragma unroll
-                for (int v = 0; v < 5; ++v)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) g[v * 3 + d] += geo[d] * (wn[v] - wc[v]);
-            }
+            for (int d = 0; d < 3; ++d) q.geo[d] = ld32_stream(m.lsqw + (size_t)(k * 3 + d) * NRp, c);
         }
+        slot_r(m, k, c, q.r);
+    };
+    // its terms, in the reference's order; the r vector goes to LDS for the
+    // limiter and the reconstruction
+    auto use_slot = [&](int k, const Slot& q) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r_lds[k * 3 + d][threadIdx.x] = q.r[d];
+        if (nbv[k] >= NRp) ghost_slots |= 1u << k;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            if (q.wn[v] > mx[v]) mx[v] = q.wn[v];
+            if (q.wn[v] < mn[v]) mn[v] = q.wn[v];
+        }
+        if (GRAD == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) g[v * 3 + d] += 0.5 * (wc[v] + q.wn[v]) * q.geo[d];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) g[v * 3 + d] += q.geo[d] * (q.wn[v] - wc[v]);
+        }
+    };
+    if (FULL) {
+        // one slot ahead: slot k + 1's loads are in flight while slot k's terms
+        // are formed; the scheduling barriers keep the compiler from hoisting
+        // further loads (more registers, fewer waves)
+        Slot q[2];
+        load_slot(0, q[0]);
+#pragma unroll
+        for (int k = 0; k < MAXS; ++k) {
+            if (k + 1 < MAXS) load_slot(k + 1, q[(k + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            use_slot(k, q[k & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < MAXS; ++k)
+            if (k < nf) {
+                Slot q;
+                load_slot(k, q);
+                use_slot(k, q);
+            }
+    }
     if (GRAD == 0) {
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] /= vol;
@@ -303,6 +329,24 @@ __global__ void __launch_bounds__(WG) k_interp(DMesh m, DState s, double k3, dou
             if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
                 ghost_face_state<VENKAT>(m, s, k, c, nbv[k], g, wc, eps2, omega);
         }
+}
+
+// Four waves per SIMD (at most 128 VGPRs): with the slot pipeline the compiler
+// would otherwise take 130 and run three, which measured no faster than the
+// serial slots; at four the Venkatakrishnan variants keep 12 bytes per lane in
+// scratch and still run 4-5 % faster than the serial slots on the 4.7M-cell
+// channel.
+template <int GRAD, bool VENKAT, int MAXS, int WG>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) k_interp(DMesh m, DState s, double k3,
+                                                                                       double omega, int ntiles) {
+    const int tile = tile_of<WG>(m, ntiles);
+    if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
+    const int c = tile * WG + threadIdx.x;
+    const int nf = ld32_stream(m.nf, c);
+    if (nf == 0) return;
+    __shared__ double r_lds[MAXS * 3][WG];  // r vectors of the cell's faces, for passes 2 and 3
+    if (__all(nf == MAXS)) interp_cell<GRAD, VENKAT, MAXS, WG, true>(m, s, k3, omega, c, nf, r_lds);
+    else interp_cell<GRAD, VENKAT, MAXS, WG, false>(m, s, k3, omega, c, nf, r_lds);
 }
 
 // FIRST (stage 0) also does what the reference runs once per iteration before
