@@ -325,6 +325,9 @@ def main():
                          "hipGraph of two iterations replayed (no per-kernel timing)")
     ap.add_argument("--tile-cells", type=int, choices=[64, 128, 256], default=256,
                     help="cells per workgroup of the stage kernels (results are the same bit for bit)")
+    ap.add_argument("--geometry", choices=["auto", "streamed"], default="auto",
+                    help="Green-Gauss slot geometry: rebuilt from per-cell node copies on tet meshes (auto) or "
+                         "streamed from the stored per-slot copies (results are the same bit for bit)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true",
@@ -372,6 +375,8 @@ def main():
     else:
         ctx.upload(mesh)
     ctx.set_tile_cells(a.tile_cells)
+    ctx.set_slot_geometry(E.GEOMETRY_AUTO if a.geometry == "auto" else E.GEOMETRY_STREAMED)
+    slot_geometry = "rebuilt from per-cell node copies" if ctx.slot_geometry_nodes() else "streamed per-slot copies"
     ctx.init_freestream()
     t_up = time.perf_counter() - t_up
     print(f"[bench] mesh {N} cells, {G} ghosts, {F} faces, {mesh.nblocks} blocks (built in {t_mesh:.1f}s, uploaded "
@@ -474,6 +479,7 @@ def main():
                                       "scrambled" if a.brick < 0 else f"brick {a.brick}"),
                    "memory_order": a.order, "launch": "hipGraph, batch API" if a.graph else "e3d_iterate",
                    "tile_cells": a.tile_cells,
+                   "slot_geometry": slot_geometry,
                    "tile_order": "automatic by mesh size (XCD-contiguous below 262,144 cells, "
                                  "runs of 4096 cells dealt to the XCDs above)"},
         "model_bytes_per_cell_stage": model_bytes,

@@ -21,6 +21,7 @@ LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1
 ARR_W, ARR_W0, ARR_CONV, ARR_DISS, ARR_GRAD, ARR_LIM, ARR_DT, ARR_SPECRAD = range(8)
 ORDER_REFERENCE, ORDER_RCM, ORDER_MORTON = 0, 1, 2
 TILES_AUTO, TILES_LINEAR, TILES_XCD_CONTIGUOUS, TILES_XCD_CHUNKS = -1, 0, 1, 2
+GEOMETRY_AUTO, GEOMETRY_STREAMED = -1, 0
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_CONNEXION = range(4)
 
 _ip = C.POINTER(C.c_int)
@@ -133,6 +134,9 @@ def lib():
             "e3d_batch_history": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Monitor), C.c_int]),
             "e3d_set_tile_order": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_set_tile_cells": (C.c_int, [C.c_void_p, C.c_int]),
+            "e3d_set_slot_geometry": (C.c_int, [C.c_void_p, C.c_int]),
+            "e3d_mesh_tet_geometry": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_uint)]),
+            "e3d_slot_geometry_nodes": (C.c_int, [C.c_void_p]),
             "e3d_set_ordering": (C.c_int, [C.c_void_p, C.c_int]),
             "e3d_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
                                          C.POINTER(C.c_void_p)]),
@@ -237,6 +241,17 @@ class Mesh:
         _check(lib().e3d_mesh_layout(self._h, b, mode, cp.ctypes.data_as(_ip), fp.ctypes.data_as(_ip)))
         return cp, fp
 
+    def tet_geometry(self, b):
+        """(node_xyz [nreal x 12], slot_code [nreal]) of block b: the per-cell node copies and
+        slot codes from which the Green-Gauss kernel rebuilds S and r on a tet mesh; raises
+        E3DError if the block holds other cells or a rebuilt slot differs from the stored metrics."""
+        n = self.block(b)["nreal"]
+        xyz = np.empty((n, 12), np.float64)
+        code = np.empty(n, np.uint32)
+        _check(lib().e3d_mesh_tet_geometry(self._h, b, xyz.ctypes.data_as(_dp),
+                                           code.ctypes.data_as(C.POINTER(C.c_uint))))
+        return xyz, code
+
     def close(self):
         if self._h and self._h.value:
             lib().e3d_mesh_free(self._h)
@@ -334,6 +349,15 @@ class Context:
     def set_tile_cells(self, cells):
         """Cells per workgroup of the stage kernels: 64, 128 or 256 (default); addresses only."""
         self._c(lib().e3d_set_tile_cells(self._h, int(cells)))
+
+    def set_slot_geometry(self, mode):
+        """GEOMETRY_AUTO (default: Green-Gauss slot geometry rebuilt from node copies on tet
+        meshes) or GEOMETRY_STREAMED (stored per-slot copies); same results either way."""
+        self._c(lib().e3d_set_slot_geometry(self._h, int(mode)))
+
+    def slot_geometry_nodes(self):
+        """True when the Green-Gauss stage kernel rebuilds the slot geometry from node copies."""
+        return bool(self._c(lib().e3d_slot_geometry_nodes(self._h)))
 
     def set_ordering(self, mode):
         """Memory order of the next upload: ORDER_REFERENCE, ORDER_RCM or ORDER_MORTON."""

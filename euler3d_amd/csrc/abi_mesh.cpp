@@ -14,6 +14,7 @@
 #include "mesh.hpp"
 #include "ordering.hpp"
 #include "mesh_internal.hpp"
+#include "tet_geometry.hpp"
 
 namespace e3d {
 thread_local std::string g_thread_error;
@@ -186,6 +187,16 @@ int e3d_mesh_layout(const e3d_mesh* m, int b, int mode, int* cell_pos, int* face
     block_layout(B, mode, cp, fp);
     std::copy(cp.begin(), cp.end(), cell_pos);
     std::copy(fp.begin(), fp.end(), face_pos);
+    return E3D_OK;
+}
+
+int e3d_mesh_tet_geometry(const e3d_mesh* m, int b, double* node_xyz, unsigned* slot_code) {
+    if (!m || b < 0 || b >= (int)m->built.size() || !node_xyz || !slot_code)
+        return set_error(nullptr, E3D_EINVAL, "e3d_mesh_tet_geometry: bad argument");
+    const int rc = tet_slot_codes(m->built[b], m->raw.blocks[b], node_xyz, slot_code);
+    if (rc == 1) return set_error(nullptr, E3D_EINVAL, "e3d_mesh_tet_geometry: the block has cells other than tets");
+    if (rc == 2)
+        return set_error(nullptr, E3D_EINVAL, "e3d_mesh_tet_geometry: a rebuilt slot differs from the stored metrics");
     return E3D_OK;
 }
 

@@ -16,6 +16,7 @@
 #include "ordering.hpp"
 #include "parallel.hpp"
 #include "physics.hpp"
+#include "tet_geometry.hpp"
 
 namespace e3d {
 void launch_save_w0(const Range&, const DMesh&, const DState&, double, hipStream_t);
@@ -33,7 +34,7 @@ void launch_smooth(const Range&, const DMesh&, const DState&, hipStream_t);
 void launch_block_monitors(int, int, const int*, const int*, const int*, const int*, const int*, const int*,
                            const double*, const DMesh&, const double*, const double*, const double*, double*, double*,
                            hipStream_t);
-void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, int, hipStream_t);
+void launch_interp_fused(const DMesh&, const DState&, int, int, double, double, int, bool, hipStream_t);
 void launch_batch_reduce(int, const int*, const double*, const void*, int*, double*, int*, int*, void*, int, int*,
                          long long*, hipStream_t);
 void launch_clock_mark(long long*, hipStream_t);
@@ -104,6 +105,7 @@ struct e3d_ctx {
     bool capturing = false;
     int tile_pref = E3D_TILES_AUTO;       // tile order asked for (e3d_set_tile_order)
     int tile_cells = 256;                 // cells per workgroup of the stage kernels (e3d_set_tile_cells)
+    int slot_geometry = E3D_GEOMETRY_AUTO;  // Green-Gauss slot geometry source (e3d_set_slot_geometry)
     int tile_map = E3D_TILES_XCD_CHUNKS;  // tile order in use (resolve_tile_order)
     int ordering = ORDER_REFERENCE;  // memory order of the next upload (e3d_set_ordering)
     // per block: device position of each real cell and face (empty: the reference's numbering)
@@ -417,6 +419,7 @@ int e3d_set_params(e3d_ctx* c, const e3d_params* p) {
 // Every block of every mesh, in upload order, with the instance it belongs to.
 struct BlockSrc {
     const HostBlock* B;
+    const RawBlock* R;
     const e3d_mesh::Flat* F;
     int ghost_count;
     int base;  // global index of the mesh's block 0
@@ -432,7 +435,7 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     for (size_t i = 0; i < meshes.size(); ++i) {
         const int base = (int)src.size();
         for (size_t b = 0; b < meshes[i]->built.size(); ++b)
-            src.push_back({&meshes[i]->built[b], &meshes[i]->flat[b], meshes[i]->raw.topo[b].ghost_count, base,
+            src.push_back({&meshes[i]->built[b], &meshes[i]->raw.blocks[b], &meshes[i]->flat[b], meshes[i]->raw.topo[b].ghost_count, base,
                            (int)i});
         c->inst_b0.push_back((int)src.size());
     }
@@ -608,6 +611,36 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         }
     }
     c->n_itf = (int)itf_dst.size();
+    // Tet meshes: per-cell node copies and slot codes from which the
+    // Green-Gauss kernel rebuilds S and r (tet_geometry.hpp); tet_slot_codes
+    // rebuilds every slot on the host with the kernel's own expressions and
+    // compares it bit for bit with the stored metrics, and the copies are
+    // kept only if every block agrees.
+    std::vector<double> cnode;
+    std::vector<unsigned> cgeo;
+    if (nslot == 4) {
+        cnode.assign(12 * NR, 0.0);
+        cgeo.assign(NR, 0u);
+        bool ok = true;
+        std::vector<double> xyz;
+        std::vector<unsigned> code;
+        for (int b = 0; b < nb && ok; ++b) {
+            const HostBlock& B = *src[b].B;
+            xyz.assign(12 * (size_t)B.nreal, 0.0);
+            code.assign(B.nreal, 0u);
+            ok = tet_slot_codes(B, *src[b].R, xyz.data(), code.data()) == 0;
+            for (int q = 0; ok && q < B.nreal; ++q) {
+                const size_t gc = (size_t)global_cell(c, b, q);
+                cgeo[gc] = code[q];
+                for (int i = 0; i < 12; ++i)  // pair plane i / 2 holds values 2(i/2), 2(i/2) + 1
+                    cnode[2 * ((size_t)(i >> 1) * NR + gc) + (i & 1)] = xyz[12 * (size_t)q + i];
+            }
+        }
+        if (!ok) {
+            cnode.clear();
+            cgeo.clear();
+        }
+    }
     // monitor reduction: enough workgroups per block that the largest block's
     // tiles and wall faces are spread over the chip (at most 64 per block)
     {
@@ -648,6 +681,18 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
     m.sRxy = reinterpret_cast<const double2*>(d_sRxy);
     m.sRz = d_sRz;
     m.mate = d_mate; m.g_r = d_gr; m.g_n = d_gn;
+    m.cnode = nullptr;
+    m.cgeo = nullptr;
+    if (!cgeo.empty()) {
+        double* d_cnode;
+        unsigned* d_cgeo;
+        if ((rc = upload(c, &d_cnode, cnode)) || (rc = upload(c, &d_cgeo, cgeo))) {
+            free_device(c);
+            return rc;
+        }
+        m.cnode = reinterpret_cast<const double2*>(d_cnode);
+        m.cgeo = d_cgeo;
+    }
     m.blk_inst = c->d_blk_inst; m.inst_fs = c->d_inst_fs; m.active = nullptr;
     resolve_tile_order(c);
     DState& s = c->s;
@@ -1006,7 +1051,7 @@ int e3d_iterate(e3d_ctx* c, int n, e3d_monitor* history) {
             if (order2) {
                 ev_begin(c, KI_INTERP, &ev);
                 launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, k * k * k, p.omega,
-                                    c->tile_cells, c->stream);
+                                    c->tile_cells, c->slot_geometry != E3D_GEOMETRY_STREAMED, c->stream);
                 ev_end(c, ev);
             }
             ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
@@ -1076,7 +1121,7 @@ static int batch_iteration(e3d_ctx* c) {
         if (order2) {
             ev_begin(c, KI_INTERP, &ev);
             launch_interp_fused(c->m, c->s, p.gradient, p.limiter == E3D_LIM_VENKATAKRISHNAN, p.k * p.k * p.k, p.omega,
-                                c->tile_cells, c->stream);
+                                c->tile_cells, c->slot_geometry != E3D_GEOMETRY_STREAMED, c->stream);
             ev_end(c, ev);
         }
         ev_begin(c, st == 0 ? KI_FLUX_UPDATE0 : c->beta[st] != 0.0 ? KI_FLUX_UPDATE : KI_FLUX_UPDATE_ND, &ev);
@@ -1227,6 +1272,21 @@ int e3d_set_tile_cells(e3d_ctx* c, int cells) {
     c->tile_cells = cells;
     drop_graph(c);
     return E3D_OK;
+}
+
+int e3d_set_slot_geometry(e3d_ctx* c, int mode) {
+    if (!c) return set_error(nullptr, E3D_EINVAL, "null context");
+    if (mode != E3D_GEOMETRY_AUTO && mode != E3D_GEOMETRY_STREAMED)
+        return set_error(c, E3D_EINVAL, "e3d_set_slot_geometry: mode must be -1 or 0");
+    c->slot_geometry = mode;
+    drop_graph(c);
+    return E3D_OK;
+}
+
+int e3d_slot_geometry_nodes(const e3d_ctx* c) {
+    if (!c) return E3D_EINVAL;
+    return c->slot_geometry != E3D_GEOMETRY_STREAMED && c->have_mesh && c->m.cnode && c->m.cgeo &&
+           c->have_params && c->params.gradient == E3D_GRAD_GREEN_GAUSS && c->params.order == 2;
 }
 
 // ------------------------------------------------------------- measurement

@@ -42,6 +42,13 @@ struct DMesh {
     const double2* sRxy;  // [nslot][NRp] the face's r vector on this cell's side
     const double* sRz;
     const int* mate;      // [nslot][NRp] face-state record of the other cell of the face; bit 31: this cell is right
+    // Tet meshes: each real cell's four node coordinates in its node order, as
+    // six (x, y)-style pair planes (x0,y0) (z0,x1) (y1,z1) (x2,y2) (z2,x3)
+    // (y3,z3), and its slot code (tet_geometry.hpp), from which the Green-Gauss
+    // kernel rebuilds S and r of every slot; null where the mesh has other cells
+    // or a slot's rebuilt values would differ from the stored ones.
+    const double2* cnode; // [6][NRp]
+    const unsigned* cgeo; // [NRp], 0 on padding cells
     const double* g_r;    // [3][NG] r vector of each ghost on its face (the reflected right vector)
     const double* g_n;    // [3][NG] unit normal S/|S| of each ghost's face (S as stored: out of the interior cell)
     // Instances: a context may hold a batch of independent meshes, each with its

@@ -30,6 +30,7 @@
 #include "device.hpp"
 #include "kcommon.hpp"
 #include "physics.hpp"
+#include "tet_geometry.hpp"
 
 namespace e3d {
 
@@ -209,10 +210,21 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
 // face slots are straight-line code whose loads the compiler can overlap;
 // otherwise slot k is skipped where k >= nf. Both forms evaluate the same
 // expressions in the same order.
-template <int GRAD, bool VENKAT, int MAXS, int WG, bool FULL>
+//
+// NODE (tet meshes, Green-Gauss): S and r of each slot are rebuilt from the
+// cell's four node coordinates and its slot code (tet_geometry.hpp), 100
+// bytes per cell, instead of streaming the stored per-slot copies, 192 bytes;
+// the doubles are the same (the upload checks every slot before it enables
+// this form). The nodes are staged in the lane's column of the r-vector LDS
+// array (12 doubles, the size of the four r vectors), S is formed from them
+// per slot in pass 1, and the r vectors then replace them for passes 2 and 3.
+// Each lane touches only its own column, so no barrier is needed.
+template <int GRAD, bool VENKAT, int MAXS, int WG, bool FULL, bool NODE>
 __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, double k3, double omega, int c, int nf,
-                                            double (*r_lds)[WG]) {
+                                            unsigned code, double (*r_lds)[WG]) {
+    static_assert(!NODE || (GRAD == 0 && MAXS == 4), "node-rebuilt geometry: Green-Gauss on tets");
     const int NRp = m.NRp, ns = m.nslot;
+    const int t = threadIdx.x;
     double wc[5];
     load_w32(s.W, m.NA, c, wc);
     const double vol = ld32_stream(m.vol, c);
@@ -225,6 +237,15 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
     int nbv[MAXS];
 #pragma unroll
     for (int k = 0; k < MAXS; ++k) nbv[k] = k < ns ? ld32_stream(m.nbr + (size_t)k * NRp, c) : c;
+    if (NODE) {  // node j's x, y, z in rows 3j, 3j+1, 3j+2; planes (x0,y0) (z0,x1) (y1,z1) (x2,y2) (z2,x3) (y3,z3)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const double2 p = ld32_stream(m.cnode + (size_t)j * NRp, c);
+            r_lds[2 * j][t] = p.x;
+            r_lds[2 * j + 1][t] = p.y;
+        }
+        asm volatile("" ::: "memory");  // read back per slot, not forwarded in registers (as below)
+    }
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = 0.0;
 #pragma unroll
@@ -234,8 +255,10 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
     };
     // the neighbour's state, S oriented away from this cell (Green-Gauss) or the
     // least-squares weights, and the r vector of face slot k
+    // (NODE: the neighbour's state only; S is formed when the slot is used)
     auto load_slot = [&](int k, Slot& q) {
         load_w32(s.W, m.NA, nbv[k], q.wn);
+        if (NODE) return;
         if (GRAD == 0) {
             slot_S(m, k, c, q.geo);
         } else {
@@ -244,11 +267,26 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
         }
         slot_r(m, k, c, q.r);
     };
+    // node i (0-2) of slot k's face, from the staged node coordinates
+    auto face_node = [&](int k, int i, double& x, double& y, double& z) {
+        const int l = (code >> (8 * k + 2 * i)) & 3;
+        x = r_lds[3 * l][t];
+        y = r_lds[3 * l + 1][t];
+        z = r_lds[3 * l + 2][t];
+    };
     // its terms, in the reference's order; the r vector goes to LDS for the
     // limiter and the reconstruction
-    auto use_slot = [&](int k, const Slot& q) {
+    auto use_slot = [&](int k, Slot& q) {
+        if (NODE) {
+            double x1, y1, z1, x2, y2, z2, x3, y3, z3;
+            face_node(k, 0, x1, y1, z1);
+            face_node(k, 1, x2, y2, z2);
+            face_node(k, 2, x3, y3, z3);
+            tri_area_vector(x1, y1, z1, x2, y2, z2, x3, y3, z3, (code >> (8 * k + 6)) & 1, q.geo);
+        } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) r_lds[k * 3 + d][threadIdx.x] = q.r[d];
+            for (int d = 0; d < 3; ++d) r_lds[k * 3 + d][t] = q.r[d];
+        }
         if (nbv[k] >= NRp) ghost_slots |= 1u << k;
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
@@ -288,6 +326,31 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
                 load_slot(k, q);
                 use_slot(k, q);
             }
+    }
+    if (NODE) {  // the r vectors of every slot replace the staged nodes (held in registers meanwhile)
+        double x[4], y[4], z[4], cc[3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x[j] = r_lds[3 * j][t];
+            y[j] = r_lds[3 * j + 1][t];
+            z[j] = r_lds[3 * j + 2][t];
+        }
+        tet_centre(x, y, z, cc);
+        double r[3 * MAXS];
+#pragma unroll
+        for (int k = 0; k < MAXS; ++k) {
+            double x1, y1, z1, x2, y2, z2, x3, y3, z3;
+            face_node(k, 0, x1, y1, z1);
+            face_node(k, 1, x2, y2, z2);
+            face_node(k, 2, x3, y3, z3);
+            tri_r_vector(x1, y1, z1, x2, y2, z2, x3, y3, z3, cc, r + 3 * k);
+        }
+#pragma unroll
+        for (int i = 0; i < 3 * MAXS; ++i) r_lds[i][t] = r[i];
+        // passes 2 and 3 read the r vectors back from LDS: without this the
+        // compiler forwards the stored values and keeps all twelve in
+        // registers through both passes (spilling to scratch)
+        asm volatile("" ::: "memory");
     }
     if (GRAD == 0) {
 #pragma unroll
@@ -331,22 +394,27 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
         }
 }
 
-// Four waves per SIMD (at most 128 VGPRs): with the slot pipeline the compiler
-// would otherwise take 130 and run three, which measured no faster than the
-// serial slots; at four the Venkatakrishnan variants keep 12 bytes per lane in
-// scratch and still run 4-5 % faster than the serial slots on the 4.7M-cell
-// channel.
-template <int GRAD, bool VENKAT, int MAXS, int WG>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) k_interp(DMesh m, DState s, double k3,
-                                                                                       double omega, int ntiles) {
+// Streamed geometry: four waves per SIMD (at most 128 VGPRs): with the slot
+// pipeline the compiler would otherwise take 130 and run three, which measured
+// no faster than the serial slots; at four the Venkatakrishnan variants keep
+// 12 bytes per lane in scratch and still run 4-5 % faster than the serial
+// slots on the 4.7M-cell channel. Node-rebuilt geometry: three waves (156
+// VGPRs, no scratch); held to four it keeps 116 bytes per lane in scratch and
+// runs 0.59 ms instead of 0.38 ms there.
+template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
+k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
     const int tile = tile_of<WG>(m, ntiles);
     if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
     const int c = tile * WG + threadIdx.x;
-    const int nf = ld32_stream(m.nf, c);
+    // NODE: the slot code stands in for the face count (zero on padding cells, a tet's 4 otherwise)
+    const unsigned code = NODE ? ld32_stream(m.cgeo, c) : 0u;
+    const int nf = NODE ? (code ? MAXS : 0) : ld32_stream(m.nf, c);
     if (nf == 0) return;
     __shared__ double r_lds[MAXS * 3][WG];  // r vectors of the cell's faces, for passes 2 and 3
-    if (__all(nf == MAXS)) interp_cell<GRAD, VENKAT, MAXS, WG, true>(m, s, k3, omega, c, nf, r_lds);
-    else interp_cell<GRAD, VENKAT, MAXS, WG, false>(m, s, k3, omega, c, nf, r_lds);
+    // (NODE: every cell left is a tet, nf == MAXS)
+    if (NODE || __all(nf == MAXS)) interp_cell<GRAD, VENKAT, MAXS, WG, true, NODE>(m, s, k3, omega, c, nf, code, r_lds);
+    else interp_cell<GRAD, VENKAT, MAXS, WG, false, NODE>(m, s, k3, omega, c, nf, code, r_lds);
 }
 
 // FIRST (stage 0) also does what the reference runs once per iteration before
@@ -495,15 +563,19 @@ __global__ void __launch_bounds__(WG) k_flux(DMesh m, DState s, Gas gas, double 
 
 // ------------------------------------------------------------ host launchers
 
+// nodes: rebuild the Green-Gauss slot geometry from the per-cell node copies
+// (m.cnode, m.cgeo; tet meshes only), else stream the stored per-slot copies.
 void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, double k3, double omega, int wg,
-                         hipStream_t st) {
+                         bool nodes, hipStream_t st) {
     const int nt = m.NRp / wg;
     const dim3 gr(xcd_grid(nt, wg)), bl(wg);
-#define IL(G, V, S, W) hipLaunchKernelGGL((k_interp<G, V, S, W>), gr, bl, 0, st, m, s, k3, omega, nt)
-#define IL_W(G, V, S) { if (wg == 64) IL(G, V, S, 64); else if (wg == 128) IL(G, V, S, 128); else IL(G, V, S, 256); }
-#define IL_S(G, V) { if (m.nslot <= 4) IL_W(G, V, 4) else IL_W(G, V, 6) }
+#define IL(G, V, S, W, N) hipLaunchKernelGGL((k_interp<G, V, S, W, N>), gr, bl, 0, st, m, s, k3, omega, nt)
+#define IL_W(G, V, S, N) { if (wg == 64) IL(G, V, S, 64, N); else if (wg == 128) IL(G, V, S, 128, N); else IL(G, V, S, 256, N); }
+#define IL_S(G, V) { if (m.nslot <= 4) IL_W(G, V, 4, false) else IL_W(G, V, 6, false) }
 #define IL_V(G) { if (venkat) IL_S(G, true) else IL_S(G, false) }
-    if (grad == 0) IL_V(0) else IL_V(1)
+    if (grad == 0 && nodes && m.nslot == 4 && m.cnode && m.cgeo) {
+        if (venkat) IL_W(0, true, 4, true) else IL_W(0, false, 4, true)
+    } else if (grad == 0) IL_V(0) else IL_V(1)
 #undef IL_V
 #undef IL_S
 #undef IL_W

@@ -272,6 +272,23 @@ int e3d_set_tile_order(e3d_ctx* ctx, int order);
  * cell whatever the tile size and the residual-norm partial sums are formed
  * per 64 cells, so results are the same bit for bit. */
 int e3d_set_tile_cells(e3d_ctx* ctx, int cells);
+/* Where the second-order Green-Gauss stage kernel takes each face slot's area
+ * vector S and r vector from. No counterpart in the reference, whose cells
+ * hold pointers to stored face metrics (Metrics/src/MetricsInitializer.cpp).
+ * E3D_GEOMETRY_AUTO (default): on a mesh of tetrahedra, rebuilt in the kernel
+ * from a per-cell copy of the cell's four node coordinates with the metric
+ * expressions of MetricsInitializer.cpp:64-142, 337-504 (100 bytes read per
+ * cell instead of 192); the upload rebuilds every slot on the host the same
+ * way and keeps this form only if all equal the stored metrics bit for bit.
+ * Other meshes, least squares, and E3D_GEOMETRY_STREAMED: the stored per-slot
+ * copies. Addresses and arithmetic only: results are the same. */
+#define E3D_GEOMETRY_AUTO (-1)
+#define E3D_GEOMETRY_STREAMED 0
+int e3d_set_slot_geometry(e3d_ctx* ctx, int mode);
+/* 1 when the Green-Gauss stage kernel of the uploaded mesh and current
+ * parameters rebuilds the slot geometry from node copies, 0 when it streams
+ * the stored copies. */
+int e3d_slot_geometry_nodes(const e3d_ctx* ctx);
 
 /* Memory order of the cells and faces of the next uploaded mesh or batch.
  * E3D_ORDER_REFERENCE (default) keeps the reference's numbering
@@ -295,6 +312,17 @@ int e3d_set_ordering(e3d_ctx* ctx, int mode);
  * permutations, 0-based within the block). Host only; for callers that keep
  * their own arrays in device order. */
 int e3d_mesh_layout(const e3d_mesh* mesh, int block, int mode, int* cell_pos, int* face_pos);
+/* Host only: the per-cell node copies and slot codes from which the
+ * Green-Gauss stage kernel rebuilds each face slot's S and r on a mesh of
+ * tetrahedra (E3D_GEOMETRY_AUTO). node_xyz[12 q + 3 j + d]: coordinate d of
+ * node j of real cell q in its element node order; slot_code[q]: one byte per
+ * face slot k (bits 8k..8k+7) with the local node indices of the face's three
+ * nodes in the face's order (2 bits each) and bit 6 set where the slot holds
+ * the metric formula's S negated. Every slot is rebuilt with the expressions
+ * of Metrics/src/MetricsInitializer.cpp:64-142, 337-504 and compared bit for
+ * bit with the block's stored metrics; E3D_EINVAL if the block holds other
+ * cells or a rebuilt value differs. No counterpart in the reference. */
+int e3d_mesh_tet_geometry(const e3d_mesh* mesh, int block, double* node_xyz, unsigned* slot_code);
 
 /* ------------------------------------------------------ learned warm start
  * A coordinate MLP that predicts an initial flow field, replacing the uniform
