@@ -352,9 +352,10 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
         // registers through both passes (spilling to scratch)
         asm volatile("" ::: "memory");
     }
-    if (GRAD == 0) {
+    if (GRAD == 0) {  // g / vol (GreenGauss.cpp:153-163); g is a sum started from +0.0
+        const double rv = 1.0 / vol;
 #pragma unroll
-        for (int i = 0; i < 15; ++i) g[i] /= vol;
+        for (int i = 0; i < 15; ++i) g[i] = div_by(g[i], vol, rv);
     }
     // pass 2: limiter over the faces, r vectors from LDS
     const double eps2 = k3 * (vol);
