@@ -138,91 +138,95 @@ __global__ void __launch_bounds__(kThreads) k_gemm(int M, int N, int ksplit, con
 }
 
 // The same product with the workgroup's operand tiles staged in LDS: a
-// 128 x 128 output tile per workgroup, 64 x 64 (4 x 4 MFMA tiles) per wave,
-// so each 16-byte operand fetched from memory feeds 16 MFMAs of the
-// workgroup instead of 2-4. A k-step covers KS = 32 or 64 columns of k; the
+// TM x T output tile per workgroup (128 x 128 or 64 x 64), a quarter of it
+// per wave (4 x 4 or 2 x 2 MFMA tiles), so each 16-byte operand fetched from
+// memory feeds up to 16 MFMAs of the workgroup instead of 2-4. A k-step covers KS = 32 or 64 columns of k; the
 // next k-step's tiles are loaded into registers while this step's MFMAs run
 // and then stored into the second of two LDS buffers, so a k-step needs one
 // barrier. Every output element accumulates over k in 32-wide MFMAs in
 // ascending order, as in k_gemm, so the two give the same bits for any KS.
-// M and N must be multiples of 128, the k range of a split a multiple of KS.
-template <int EPI, int KS>
+// M and N must be multiples of TM and T, the k range of a split a multiple of KS.
+template <int EPI, int KS, int TM, int T>
 __global__ void __launch_bounds__(kThreads) k_gemm_lds(int M, int N, int ksplit, const bf16_t* __restrict__ A,
                                                        int lda, const bf16_t* __restrict__ B, int ldb, EpiArgs e) {
-    constexpr int T = 128, LD = KS + 8;   // tile; LDS row stride in bf16 (a bank-conflict-free ds_read_b128 pattern)
+    constexpr int LD = KS + 8;            // LDS row stride in bf16 (a bank-conflict-free ds_read_b128 pattern)
+    constexpr int IM = TM / 32;           // 16-row MFMA tiles per wave (the waves split the TM rows in two)
+    constexpr int JN = T / 32;            // 16-column MFMA tiles per wave (and the T columns in two)
     constexpr int CPR = KS / 8;           // 16-byte chunks per tile row and k-step
-    constexpr int NCH = T * CPR / kThreads;  // chunks per thread and operand: 2 or 4
-    __shared__ __align__(16) bf16_t smem[4 * T * LD];  // A buffers 0, 1, then B buffers 0, 1
+    constexpr int NCA = TM * CPR / kThreads, NCB = T * CPR / kThreads;  // chunks per thread: A, B
+    __shared__ __align__(16) bf16_t smem[2 * (TM + T) * LD];  // A buffers 0, 1, then B buffers 0, 1
+    bf16_t* const Bs = smem + 2 * TM * LD;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int mt = blockIdx.x * T, nt = blockIdx.y * T;
+    const int mt = blockIdx.x * TM, nt = blockIdx.y * T;
     const int k0 = blockIdx.z * ksplit, k1 = k0 + ksplit;
     const int r = lane & 15, g = lane >> 4;
     // this thread's chunks of each operand tile: chunk c is row (tid + 256 c) / CPR,
     // columns 8 ((tid + 256 c) % CPR) .. + 7 of the k-step
     const int part = (tid % CPR) * 8;
-    int lofs[NCH];
-    const bf16_t* ga_p[NCH];
-    const bf16_t* gb_p[NCH];
+    int lofa[NCA], lofb[NCB];
+    const bf16_t* ga_p[NCA];
+    const bf16_t* gb_p[NCB];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
+    for (int c = 0; c < NCA; ++c) {
         const int row = (tid + kThreads * c) / CPR;
-        lofs[c] = row * LD + part;
+        lofa[c] = row * LD + part;
         ga_p[c] = A + (size_t)(mt + row) * lda + part;
+    }
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+        const int row = (tid + kThreads * c) / CPR;
+        lofb[c] = row * LD + part;
         gb_p[c] = B + (size_t)(nt + row) * ldb + part;
     }
-    floatx4 acc[4][4];
+    floatx4 acc[IM][JN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    bf16x8 ga[NCH], gb[NCH];
+        for (int j = 0; j < JN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 ga[NCA], gb[NCB];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k0);
-        gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k0);
-    }
+    for (int c = 0; c < NCA; ++c) ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k0);
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        *reinterpret_cast<bf16x8*>(smem + lofs[c]) = ga[c];
-        *reinterpret_cast<bf16x8*>(smem + 2 * T * LD + lofs[c]) = gb[c];
-    }
+    for (int c = 0; c < NCB; ++c) gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k0);
+#pragma unroll
+    for (int c = 0; c < NCA; ++c) *reinterpret_cast<bf16x8*>(smem + lofa[c]) = ga[c];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) *reinterpret_cast<bf16x8*>(Bs + lofb[c]) = gb[c];
     __syncthreads();
     int cur = 0;
     for (int k = k0; k < k1; k += KS) {
         const bool more = k + KS < k1;  // uniform
         if (more) {  // the next k-step's tiles, in flight while this step's MFMAs run
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k + KS);
-                gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k + KS);
-            }
+            for (int c = 0; c < NCA; ++c) ga[c] = *reinterpret_cast<const bf16x8*>(ga_p[c] + k + KS);
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) gb[c] = *reinterpret_cast<const bf16x8*>(gb_p[c] + k + KS);
         }
-        const bf16_t* Ac = smem + cur * T * LD;
-        const bf16_t* Bc = smem + (2 + cur) * T * LD;
+        const bf16_t* Ac = smem + cur * TM * LD;
+        const bf16_t* Bc = Bs + cur * T * LD;
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 32) {
-            bf16x8 a[4], b[4];
+            bf16x8 a[IM], b[JN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = *reinterpret_cast<const bf16x8*>(Ac + (wm * 64 + 16 * i + r) * LD + kk + 8 * g);
+            for (int i = 0; i < IM; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(Ac + (wm * (TM / 2) + 16 * i + r) * LD + kk + 8 * g);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                b[j] = *reinterpret_cast<const bf16x8*>(Bc + (wn * 64 + 16 * j + r) * LD + kk + 8 * g);
+            for (int j = 0; j < JN; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(Bc + (wn * (T / 2) + 16 * j + r) * LD + kk + 8 * g);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < IM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < JN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if (more) {  // into the other buffer, which no wave reads during this step
-            bf16_t* An = smem + (cur ^ 1) * T * LD;
-            bf16_t* Bn = smem + (2 + (cur ^ 1)) * T * LD;
+            bf16_t* An = smem + (cur ^ 1) * TM * LD;
+            bf16_t* Bn = Bs + (cur ^ 1) * T * LD;
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                *reinterpret_cast<bf16x8*>(An + lofs[c]) = ga[c];
-                *reinterpret_cast<bf16x8*>(Bn + lofs[c]) = gb[c];
-            }
+            for (int c = 0; c < NCA; ++c) *reinterpret_cast<bf16x8*>(An + lofa[c]) = ga[c];
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) *reinterpret_cast<bf16x8*>(Bn + lofb[c]) = gb[c];
         }
         // this step's reads of buffer cur and writes of the other are done before
         // the next step reads the other and (one step later) overwrites cur
@@ -230,10 +234,10 @@ __global__ void __launch_bounds__(kThreads) k_gemm_lds(int M, int N, int ksplit,
         cur ^= 1;
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            epilogue<EPI>(e, M, N, mt + wm * 64 + 16 * i + 4 * g, nt + wn * 64 + 16 * j + r, acc[i][j]);
+        for (int j = 0; j < JN; ++j)
+            epilogue<EPI>(e, M, N, mt + wm * (TM / 2) + 16 * i + 4 * g, nt + wn * (T / 2) + 16 * j + r, acc[i][j]);
 }
 
 // Mini-batch assembly: features (padded with zeros to 32) as bf16, row-major and
@@ -476,19 +480,27 @@ void gemm(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda,
 
 // Wider column tiles (4 MFMA tiles per wave) where N allows: each A operand
 // loaded feeds twice as many MFMAs. The accumulation order of every output
-// element is the same, so results are unchanged.
+// element is the same, so results are unchanged. Tile shapes, measured on the
+// 5 x 512 network at 8192 samples (ms per training step, same bits
+// throughout): products with an epilogue (forward SiLU, input gradient) in
+// 64 x 64 tiles with a 64-wide k-step, so eight workgroups share a CU and one
+// workgroup's epilogue overlaps another's main loop (128 x 128 tiles with a
+// 32-wide k-step 0.357, 64 x 128 0.327, 128 x 64 0.320, 64 x 64 0.312, with a
+// 64-wide k-step 0.300); the weight-gradient products in 128 x 128 tiles with
+// a 64-wide k-step (64 x 64 there: 0.310-0.320).
 template <int EPI>
 void gemm_n(e3d_mlp* m, int M, int N, int K, int splits, const bf16_t* A, int lda, const bf16_t* B, int ldb,
             const EpiArgs& e) {
-    // a 64-wide k-step (half the barriers) pays off for the weight-gradient
-    // products, whose plain-store epilogue leaves the main loop as the cost;
-    // the SiLU-epilogue products measured faster with 32
-    if (EPI == EPI_STORE && M % 128 == 0 && N % 128 == 0 && (K / splits) % 64 == 0)
-        hipLaunchKernelGGL((k_gemm_lds<EPI, 64>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
-                           K / splits, A, lda, B, ldb, e);
-    else if (M % 128 == 0 && N % 128 == 0)
-        hipLaunchKernelGGL((k_gemm_lds<EPI, 32>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream, M, N,
-                           K / splits, A, lda, B, ldb, e);
+    const int ks = K / splits;
+    if (EPI == EPI_STORE && M % 128 == 0 && N % 128 == 0 && ks % 64 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 64, 128, 128>), dim3(M / 128, N / 128, splits), dim3(kThreads), 0, m->stream,
+                           M, N, ks, A, lda, B, ldb, e);
+    else if (EPI != EPI_STORE && M % 64 == 0 && N % 64 == 0 && ks % 64 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 64, 64, 64>), dim3(M / 64, N / 64, splits), dim3(kThreads), 0, m->stream, M, N,
+                           ks, A, lda, B, ldb, e);
+    else if (EPI != EPI_STORE && M % 64 == 0 && N % 64 == 0)
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 32, 64, 64>), dim3(M / 64, N / 64, splits), dim3(kThreads), 0, m->stream, M, N,
+                           ks, A, lda, B, ldb, e);
     else if (N % 64 == 0) gemm<EPI, 4>(m, M, N, K, splits, A, lda, B, ldb, e);
     else gemm<EPI, 2>(m, M, N, K, splits, A, lda, B, ldb, e);
 }
