@@ -381,7 +381,7 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
                 if (phi < lim[v]) lim[v] = phi;
             }
         }
-    // pass 3: face states of the cell; ghosts hanging off it
+    // pass 3: face states of the cell
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
         if (k < nf) {
@@ -390,18 +390,25 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
             for (int d = 0; d < 3; ++d) r[d] = r_lds[k * 3 + d][threadIdx.x];
             face_state(wc, lim, g, r, out);
             store_own_rec(s.Wf, rec_count(m), NRp, k, c, out);
+        }
+    // ghosts hanging off the cell, in a loop of their own that is not unrolled:
+    // one copy of the reflected-gradient and limiter code instead of one per
+    // slot keeps the streamed-geometry kernels within four waves' registers
+    // without scratch (on the 4.7M-cell channel within 1-4 % of the unrolled
+    // form's time, box to box)
+    if (ghost_slots) {
+#pragma unroll 1
+        for (int k = 0; k < MAXS; ++k)
             if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
                 ghost_face_state<VENKAT>(m, s, k, c, nbv[k], g, wc, eps2, omega);
-        }
+    }
 }
 
-// Streamed geometry: four waves per SIMD (at most 128 VGPRs): with the slot
-// pipeline the compiler would otherwise take 130 and run three, which measured
-// no faster than the serial slots; at four the Venkatakrishnan variants keep
-// 12 bytes per lane in scratch and still run 4-5 % faster than the serial
-// slots on the 4.7M-cell channel. Node-rebuilt geometry: three waves (156
-// VGPRs, no scratch); held to four it keeps 116 bytes per lane in scratch and
-// runs 0.59 ms instead of 0.38 ms there.
+// Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-122 used, no
+// scratch): with the slot pipeline the compiler would otherwise take more and
+// run three, which measured no faster than the serial slots. Node-rebuilt
+// geometry: three waves (146 VGPRs, no scratch); held to four it spills to
+// scratch and runs 0.59 ms instead of 0.38 ms on the 4.7M-cell channel.
 template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
 k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
