@@ -254,6 +254,53 @@ def stream_rate(table, write_fraction):
     return pts[-1][1]
 
 
+def scrambled_leg(a):
+    """The workload's mesh with its tets in pseudo-random file order (no
+    locality in the file's numbering, as from a mesher that does not reorder),
+    uploaded in the Morton memory order, against the generator's numbering in
+    file order, both with AUSM + least squares + Barth-Jespersen: with
+    Venkatakrishnan's limiter the reference method stops on the scrambled
+    numbering within three iterations (1 / d2 overflows for a subnormal d2,
+    Venkatakrishnan.cpp:227-238)."""
+    import euler3d_amd as E
+    t0 = time.perf_counter()
+    sc = SCHEMES["ausm-lsq-bj"]
+    params = E.make_params(sc["mach"], sc["aoa"], flux=sc["flux"], order=2, gradient=sc["gradient"],
+                           limiter=sc["limiter"], k=sc["k"], omega=sc["omega"], stages=5, max_iter=10 ** 9,
+                           criterion=-1e9)
+    runs = {}
+    for label, brick, order in (("generator numbering, file order", 4, E.ORDER_REFERENCE),
+                                ("pseudo-random numbering (seed 5), Morton order", -5, E.ORDER_MORTON)):
+        mesh = E.Mesh.bump(a.nx, a.ny, a.nz, 0.1, brick, tuple(a.parts))
+        ctx = E.Context(0)
+        ctx.set_params(params)
+        ctx.set_ordering(order)
+        ctx.upload(mesh)
+        ctx.set_tile_cells(a.tile_cells)
+        ctx.init_freestream()
+        ctx.iterate(3, history=False)
+        ctx.op("sync")
+        E.lib().e3d_enable_timing(ctx._h, 1)
+        steps = 10
+        t1 = time.perf_counter()
+        ctx.iterate(steps, history=False)
+        ctx.op("sync")
+        wall = time.perf_counter() - t1
+        n = sum(mesh.block(b)["nreal"] for b in range(mesh.nblocks))
+        runs[label] = {"ms_per_step": wall / steps * 1e3, "value": n * 5 * steps / wall / 1e6,
+                       "kernels_avg_ms": {k: round(ctx.kernel_time(k)[0], 4)
+                                          for k in ("interp", "flux_update0", "flux_update", "flux_update_nodiss")}}
+        ctx.close()
+        mesh.close()
+    ms = [r["ms_per_step"] for r in runs.values()]
+    out = {"workload": f"bump channel {a.nx}x{a.ny}x{a.nz}x6 tets in {a.parts} blocks, {sc['label']}, 5-stage RK",
+           "runs": runs, "scrambled_over_generator": ms[1] / ms[0], "steps": 10, "warmup": 3,
+           "wall_seconds": time.perf_counter() - t0}
+    print(f"[bench] scrambled mesh, Morton order: {ms[1]:.3f} ms per iteration, {ms[1] / ms[0]:.2f}x the generator "
+          f"numbering's {ms[0]:.3f} ms (AUSM/LSQ/BJ)", file=sys.stderr, flush=True)
+    return out
+
+
 # Warm-start leg of the bench line: the learned warm start against the cold
 # (freestream) start on a configuration that fits the bench's time budget.
 WARM_START = dict(nx=48, n_train=256, train_criterion=-6.0, n_heldout=16, criterion=-8.0, max_iter=30000,
@@ -328,6 +375,9 @@ def main():
     ap.add_argument("--geometry", choices=["auto", "streamed"], default="auto",
                     help="Green-Gauss slot geometry: rebuilt from per-cell node copies on tet meshes (auto) or "
                          "streamed from the stored per-slot copies (results are the same bit for bit)")
+    ap.add_argument("--no-scrambled", action="store_true",
+                    help="skip the scrambled-mesh leg (the same mesh with its cells in pseudo-random file order, "
+                         "Morton memory order)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true",
@@ -460,6 +510,10 @@ def main():
         hip.hipFree(dst)
         print(f"[bench] calibration copy of 1 GiB: {'done' if ok else 'FAILED'}", file=sys.stderr, flush=True)
 
+    scrambled = None
+    if a.mesh == "bump" and a.brick >= 0 and not a.no_scrambled:
+        scrambled = scrambled_leg(a)
+
     cpu, cpu1, cpu_ref = None, None, None
     if not a.no_cpu_baseline:
         cpu, cpu1 = cpu_baselines(a, sc, blocks)
@@ -487,6 +541,7 @@ def main():
         "roofline": rooflines[dominant] if dominant else None,
         "rooflines": rooflines,
         "kernels": kt,
+        "scrambled_mesh": scrambled,
         "cpu_baseline": cpu,
         "cpu_baseline_1core": cpu1,
         "cpu_baseline_reference": cpu_ref,
