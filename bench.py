@@ -11,9 +11,11 @@ HBM before the timed region starts.
 
 metric: Mcell-RK-stage/s = real cells x RK stages x steps / wall time.
 roofline: the kernel with the largest share of the timed region (interp:
-gradient + limiter + face states, or flux_update: residual + RK update + BCs
-on a stage with dissipation), algorithmic bytes per launch / average launch
-time from HIP events recorded on the context's stream during the timed region;
+gradient + limiter + face states, with the ghosts' face states in a second
+small kernel when the slot geometry is rebuilt from node copies, the two timed
+as one; or flux_update: residual + RK update + BCs on a stage with
+dissipation), algorithmic bytes per launch / average launch time from HIP
+events recorded on the context's stream during the timed region;
 "rooflines" carries the same figures for both kernels.
 cpu_baseline: the oracle (plain-C restatement of the reference's solver loop)
 on the host cores, on a bounded sample of the same workload: all 8 blocks, one

@@ -357,6 +357,17 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] = div_by(g[i], vol, rv);
     }
+    // ghosts hanging off the cell, node-geometry form: their face states are
+    // formed by k_ghost_states, one thread per ghost, from this cell's gradient
+    if (NODE && ghost_slots) {
+#pragma unroll 1
+        for (int k = 0; k < MAXS; ++k)
+            if (ghost_slots & (1u << k)) {
+                const int gi = nbv[k] - NRp;
+#pragma unroll
+                for (int i = 0; i < 15; ++i) s.gbuf[(size_t)i * m.NG + gi] = g[i];
+            }
+    }
     // pass 2: limiter over the faces, r vectors from LDS
     const double eps2 = k3 * (vol);
     double lim[5];
@@ -366,21 +377,27 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
         mn[v] -= wc[v];
         lim[v] = __builtin_inf();
     }
+    // (the node-geometry form one slot at a time, for its register budget)
+#define E3D_LIMIT_SLOT(k)                                                                   \
+    {                                                                                       \
+        double r[3];                                                                        \
+        _Pragma("unroll") for (int d = 0; d < 3; ++d) r[d] = r_lds[(k) * 3 + d][threadIdx.x]; \
+        _Pragma("unroll") for (int v = 0; v < 5; ++v) {                                     \
+            double d2 = 0.0;                                                                \
+            _Pragma("unroll") for (int d = 0; d < 3; ++d) d2 += g[v * 3 + d] * r[d];        \
+            const double phi = face_limiter<VENKAT>(d2, mx[v], mn[v], eps2, omega);         \
+            if (phi < lim[v]) lim[v] = phi;                                                 \
+        }                                                                                   \
+    }
+    if (NODE) {
+#pragma unroll 1
+        for (int k = 0; k < MAXS; ++k) E3D_LIMIT_SLOT(k)
+    } else {
 #pragma unroll
-    for (int k = 0; k < MAXS; ++k)
-        if (k < nf) {
-            double r[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) r[d] = r_lds[k * 3 + d][threadIdx.x];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                double d2 = 0.0;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) d2 += g[v * 3 + d] * r[d];
-                const double phi = face_limiter<VENKAT>(d2, mx[v], mn[v], eps2, omega);
-                if (phi < lim[v]) lim[v] = phi;
-            }
-        }
+        for (int k = 0; k < MAXS; ++k)
+            if (k < nf) E3D_LIMIT_SLOT(k)
+    }
+#undef E3D_LIMIT_SLOT
     // pass 3: face states of the cell
 #pragma unroll
     for (int k = 0; k < MAXS; ++k)
@@ -391,12 +408,11 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
             face_state(wc, lim, g, r, out);
             store_own_rec(s.Wf, rec_count(m), NRp, k, c, out);
         }
-    // ghosts hanging off the cell, in a loop of their own that is not unrolled:
-    // one copy of the reflected-gradient and limiter code instead of one per
-    // slot keeps the streamed-geometry kernels within four waves' registers
-    // without scratch (on the 4.7M-cell channel within 1-4 % of the unrolled
-    // form's time, box to box)
-    if (ghost_slots) {
+    // ghosts hanging off the cell, streamed-geometry form: in a loop of their
+    // own that is not unrolled (one copy of the reflected-gradient and limiter
+    // code instead of one per slot keeps these kernels within four waves'
+    // registers without scratch)
+    if (!NODE && ghost_slots) {
 #pragma unroll 1
         for (int k = 0; k < MAXS; ++k)
             if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
@@ -404,11 +420,35 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
     }
 }
 
+// Face state of every ghost (its record), after k_interp: the reflected (or
+// zero) gradient of its interior cell, the ghost's limiter over that single
+// neighbour and W + (lim*grad).r, by ghost_face_state with the interior
+// cell's gradient (left in gbuf by k_interp), state and eps2 = K^3 V of the
+// interior cell, the values k_interp holds for it, so the records are the
+// same doubles. Kept out of k_interp: in a wavefront only the lanes of
+// boundary cells would run it, and its registers would hold the
+// node-geometry form to three waves per SIMD instead of four.
+template <bool VENKAT>
+__global__ void __launch_bounds__(kWG) k_ghost_states(DMesh m, DState s, double k3, double omega) {
+    const int gi = blockIdx.x * kWG + threadIdx.x;
+    if (gi >= m.NG) return;
+    const int c = m.g_in[gi];
+    if (!instance_active(m, cell_instance(m, c))) return;
+    double g[15], wc[5];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) g[i] = s.gbuf[(size_t)i * m.NG + gi];
+    load_w(s.W, m.NA, c, wc);
+    const double eps2 = k3 * (m.vol[c]);
+    ghost_face_state<VENKAT>(m, s, 0, c, m.NRp + gi, g, wc, eps2, omega);
+}
+
 // Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-122 used, no
 // scratch): with the slot pipeline the compiler would otherwise take more and
 // run three, which measured no faster than the serial slots. Node-rebuilt
-// geometry: three waves (146 VGPRs, no scratch); held to four it spills to
-// scratch and runs 0.59 ms instead of 0.38 ms on the 4.7M-cell channel.
+// geometry: three waves (137-147 VGPRs, no scratch). Held to four it spills:
+// 0.59 ms instead of 0.38 ms on the 4.7M-cell channel with the ghost records
+// inside the kernel, 0.44 ms against 0.37 ms at three waves with them moved
+// to k_ghost_states (which took the three-wave form from 0.375 to 0.368 ms).
 template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
 k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
@@ -581,13 +621,19 @@ void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, 
 #define IL_W(G, V, S, N) { if (wg == 64) IL(G, V, S, 64, N); else if (wg == 128) IL(G, V, S, 128, N); else IL(G, V, S, 256, N); }
 #define IL_S(G, V) { if (m.nslot <= 4) IL_W(G, V, 4, false) else IL_W(G, V, 6, false) }
 #define IL_V(G) { if (venkat) IL_S(G, true) else IL_S(G, false) }
-    if (grad == 0 && nodes && m.nslot == 4 && m.cnode && m.cgeo) {
+    const bool node_form = grad == 0 && nodes && m.nslot == 4 && m.cnode && m.cgeo;
+    if (node_form) {
         if (venkat) IL_W(0, true, 4, true) else IL_W(0, false, 4, true)
     } else if (grad == 0) IL_V(0) else IL_V(1)
 #undef IL_V
 #undef IL_S
 #undef IL_W
 #undef IL
+    if (node_form && m.NG > 0) {
+        const dim3 gg((m.NG + kWG - 1) / kWG), gb(kWG);
+        if (venkat) hipLaunchKernelGGL((k_ghost_states<true>), gg, gb, 0, st, m, s, k3, omega);
+        else hipLaunchKernelGGL((k_ghost_states<false>), gg, gb, 0, st, m, s, k3, omega);
+    }
 }
 
 void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, bool first,
