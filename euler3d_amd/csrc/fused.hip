@@ -425,9 +425,11 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
 // neighbour and W + (lim*grad).r, by ghost_face_state with the interior
 // cell's gradient (left in gbuf by k_interp), state and eps2 = K^3 V of the
 // interior cell, the values k_interp holds for it, so the records are the
-// same doubles. Kept out of k_interp: in a wavefront only the lanes of
-// boundary cells would run it, and its registers would hold the
-// node-geometry form to three waves per SIMD instead of four.
+// same doubles. Kept out of the node-geometry k_interp: there only the
+// lanes of boundary cells would run it while the rest of the wavefront
+// waits, and its registers count against every lane (on the 4.7M-cell
+// channel the two kernels take 0.368 ms per stage, k_interp with the ghosts
+// inside 0.375 ms).
 template <bool VENKAT>
 __global__ void __launch_bounds__(kWG) k_ghost_states(DMesh m, DState s, double k3, double omega) {
     const int gi = blockIdx.x * kWG + threadIdx.x;
