@@ -1,9 +1,8 @@
 // Golden-vector dumper. Container-only test tooling: links the reference's own
-// classes (built from /root/reference by oracle/Makefile into _ref/) and drives
-// them exactly as the reference's main.cpp:17-101 and Solver::solve
-// (Solver/src/Solver.cpp:36-98) do, writing raw arrays at chosen points.
-// Never part of the product library and never shipped to the GPU box as a tool
-// the product calls.
+// classes (built from the reference's sources by oracle/Makefile into _ref/)
+// and drives them exactly as the reference's main.cpp:17-101 and
+// Solver::solve (Solver/src/Solver.cpp:36-98) do, writing raw arrays at chosen
+// points. Never part of the product library, and nothing the product calls.
 //
 // usage: ref_dump <input_file> <out_dir> <iters> <stage_iter>
 //   geometry of every block                 -> <out>/geom_b<k>_<name>.<f64|i32>
