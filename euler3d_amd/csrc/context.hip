@@ -701,7 +701,6 @@ static int upload_meshes(e3d_ctx* c, const std::vector<const e3d_mesh*>& meshes)
         (rc = dalloc(c, &s.conv, 5 * NR)) || (rc = dalloc(c, &s.diss, 5 * NR)) || (rc = dalloc(c, &s.grad, 15 * NA)) ||
         (rc = dalloc(c, &s.lim, 5 * NA)) || (rc = dalloc(c, &s.Wf, 5 * (NS + NG))) || (rc = dalloc(c, &s.dt, NR)) ||
         (rc = dalloc(c, &s.specrad, NR)) || (rc = dalloc(c, &s.part, 8 * (NR / 64 + 1))) ||
-        (rc = dalloc(c, &s.gbuf, 15 * (size_t)m.NG)) ||
         (rc = dalloc(c, &c->d_mon, 16 * (size_t)nb)) ||
         (rc = dalloc(c, &c->d_mpart, 10 * (size_t)nb * c->mon_split))) {
         free_device(c);

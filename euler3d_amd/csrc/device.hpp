@@ -84,7 +84,6 @@ struct DState {
     double* specrad;      // [NRp]
     double* part;         // [NRp/64][8] per-wavefront partial sums of the residual monitor
     double* jac;          // [10][NRp] Jacobi values of the residual smoother (allocated on first use)
-    double* gbuf;         // [15][NG] fused path: gradient of each ghost's interior cell, k_interp -> k_ghost_states
 };
 
 // Per-instance constants of a batched run's monitors (PostProcessing/src/PostProcessing.cpp:222-250,

@@ -205,6 +205,24 @@ __device__ __forceinline__ void ghost_face_state(const DMesh& m, const DState& s
     }
 }
 
+// Position of the j-th set bit (j from 0) of a 64-bit lane mask, j < popcount(mask).
+__device__ __forceinline__ int nth_set_bit(unsigned long long mask, int j) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long low = (1ull << w) - 1;
+        const int n = __popcll(mask & low);
+        if (j >= n) {
+            j -= n;
+            mask >>= w;
+            pos += w;
+        } else {
+            mask &= low;
+        }
+    }
+    return pos;
+}
+
 // One cell of k_interp. FULL: every cell of the wavefront has MAXS faces (a
 // tet mesh with MAXS = 4 always does), so the per-slot tests drop out and the
 // face slots are straight-line code whose loads the compiler can overlap;
@@ -357,17 +375,6 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
 #pragma unroll
         for (int i = 0; i < 15; ++i) g[i] = div_by(g[i], vol, rv);
     }
-    // ghosts hanging off the cell, node-geometry form: their face states are
-    // formed by k_ghost_states, one thread per ghost, from this cell's gradient
-    if (NODE && ghost_slots) {
-#pragma unroll 1
-        for (int k = 0; k < MAXS; ++k)
-            if (ghost_slots & (1u << k)) {
-                const int gi = nbv[k] - NRp;
-#pragma unroll
-                for (int i = 0; i < 15; ++i) s.gbuf[(size_t)i * m.NG + gi] = g[i];
-            }
-    }
     // pass 2: limiter over the faces, r vectors from LDS
     const double eps2 = k3 * (vol);
     double lim[5];
@@ -408,49 +415,72 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
             face_state(wc, lim, g, r, out);
             store_own_rec(s.Wf, rec_count(m), NRp, k, c, out);
         }
-    // ghosts hanging off the cell, streamed-geometry form: in a loop of their
-    // own that is not unrolled (one copy of the reflected-gradient and limiter
-    // code instead of one per slot keeps these kernels within four waves'
-    // registers without scratch)
-    if (!NODE && ghost_slots) {
-#pragma unroll 1
-        for (int k = 0; k < MAXS; ++k)
-            if (ghost_slots & (1u << k))  // a ghost's volume is its interior cell's
-                ghost_face_state<VENKAT>(m, s, k, c, nbv[k], g, wc, eps2, omega);
-    }
-}
-
-// Face state of every ghost (its record), after k_interp: the reflected (or
-// zero) gradient of its interior cell, the ghost's limiter over that single
-// neighbour and W + (lim*grad).r, by ghost_face_state with the interior
-// cell's gradient (left in gbuf by k_interp), state and eps2 = K^3 V of the
-// interior cell, the values k_interp holds for it, so the records are the
-// same doubles. Kept out of the node-geometry k_interp: there only the
-// lanes of boundary cells would run it while the rest of the wavefront
-// waits, and its registers count against every lane (on the 4.7M-cell
-// channel the two kernels take 0.368 ms per stage, k_interp with the ghosts
-// inside 0.375 ms).
-template <bool VENKAT>
-__global__ void __launch_bounds__(kWG) k_ghost_states(DMesh m, DState s, double k3, double omega) {
-    const int gi = blockIdx.x * kWG + threadIdx.x;
-    if (gi >= m.NG) return;
-    const int c = m.g_in[gi];
-    if (!instance_active(m, cell_instance(m, c))) return;
-    double g[15], wc[5];
+    // Ghosts hanging off the cells of this wavefront: their face states, by
+    // ghost_face_state with the interior cell's gradient, state and eps2 (a
+    // ghost's volume is its interior cell's). The wavefront's ghost slots are
+    // dealt to its active lanes in turn (slot 0's ghosts in lane order, then
+    // slot 1's, ...), each lane fetching its task's operands from the interior
+    // cell's lane (the gradient through that lane's LDS column, free after
+    // pass 3, the rest by lane shuffles), so one pass serves every ghost of
+    // the wavefront instead of one divergent pass per face slot. Same
+    // operands, same records.
+    {
+        const int lane = __lane_id();
+        const unsigned long long act = __ballot(1);
+        const int rank = __popcll(act & ((1ull << lane) - 1)), nact = __popcll(act);
+        unsigned long long slot_mask[MAXS];
+        int first[MAXS + 1];
+        first[0] = 0;
 #pragma unroll
-    for (int i = 0; i < 15; ++i) g[i] = s.gbuf[(size_t)i * m.NG + gi];
-    load_w(s.W, m.NA, c, wc);
-    const double eps2 = k3 * (m.vol[c]);
-    ghost_face_state<VENKAT>(m, s, 0, c, m.NRp + gi, g, wc, eps2, omega);
+        for (int k = 0; k < MAXS; ++k) {
+            slot_mask[k] = __ballot((ghost_slots >> k) & 1u);
+            first[k + 1] = first[k] + __popcll(slot_mask[k]);
+        }
+        const int ntask = first[MAXS];  // the same on every lane
+        if (ntask > 0) {
+#pragma unroll
+            for (int i = 0; i < 15; ++i) r_lds[i][t] = g[i];
+            // the stores above before the other lanes' loads below (LDS
+            // operations of one wavefront complete in order)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll 1
+        for (int t0 = 0; t0 < ntask; t0 += nact) {
+            const int task = t0 + rank;
+            int k = 0;
+#pragma unroll
+            for (int kk = 1; kk < MAXS; ++kk)
+                if (task >= first[kk]) k = kk;
+            unsigned long long msk = slot_mask[0];
+#pragma unroll
+            for (int kk = 1; kk < MAXS; ++kk)
+                if (k == kk) msk = slot_mask[kk];
+            const bool mine = task < ntask;
+            const int src = mine ? nth_set_bit(msk, task - first[k]) : lane;
+            double gs[15], ws[5];
+#pragma unroll
+            for (int i = 0; i < 15; ++i) gs[i] = r_lds[i][t - lane + src];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) ws[v] = __shfl(wc[v], src, 64);
+            const double es = __shfl(eps2, src, 64);
+            int nb = __shfl(nbv[0], src, 64);
+#pragma unroll
+            for (int kk = 1; kk < MAXS; ++kk) {
+                const int x = __shfl(nbv[kk], src, 64);
+                if (k == kk) nb = x;
+            }
+            if (mine) ghost_face_state<VENKAT>(m, s, k, c - lane + src, nb, gs, ws, es, omega);
+        }
+    }
 }
 
 // Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-122 used, no
 // scratch): with the slot pipeline the compiler would otherwise take more and
 // run three, which measured no faster than the serial slots. Node-rebuilt
-// geometry: three waves (137-147 VGPRs, no scratch). Held to four it spills:
-// 0.59 ms instead of 0.38 ms on the 4.7M-cell channel with the ghost records
-// inside the kernel, 0.44 ms against 0.37 ms at three waves with them moved
-// to k_ghost_states (which took the three-wave form from 0.375 to 0.368 ms).
+// geometry: three waves (no scratch); held to four it spills to scratch and
+// runs 0.59 ms instead of 0.38 ms on the 4.7M-cell channel.
 template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
 k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
@@ -461,7 +491,8 @@ k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
     const unsigned code = NODE ? ld32_stream(m.cgeo, c) : 0u;
     const int nf = NODE ? (code ? MAXS : 0) : ld32_stream(m.nf, c);
     if (nf == 0) return;
-    __shared__ double r_lds[MAXS * 3][WG];  // r vectors of the cell's faces, for passes 2 and 3
+    // r vectors of the cell's faces for passes 2 and 3, then its gradient for the ghost pass
+    __shared__ double r_lds[MAXS * 3 > 15 ? MAXS * 3 : 15][WG];
     // (NODE: every cell left is a tet, nf == MAXS)
     if (NODE || __all(nf == MAXS)) interp_cell<GRAD, VENKAT, MAXS, WG, true, NODE>(m, s, k3, omega, c, nf, code, r_lds);
     else interp_cell<GRAD, VENKAT, MAXS, WG, false, NODE>(m, s, k3, omega, c, nf, code, r_lds);
@@ -631,11 +662,6 @@ void launch_interp_fused(const DMesh& m, const DState& s, int grad, int venkat, 
 #undef IL_S
 #undef IL_W
 #undef IL
-    if (node_form && m.NG > 0) {
-        const dim3 gg((m.NG + kWG - 1) / kWG), gb(kWG);
-        if (venkat) hipLaunchKernelGGL((k_ghost_states<true>), gg, gb, 0, st, m, s, k3, omega);
-        else hipLaunchKernelGGL((k_ghost_states<false>), gg, gb, 0, st, m, s, k3, omega);
-    }
 }
 
 void launch_flux_update(const DMesh& m, const DState& s, int flux, bool order2, bool diss, bool last, bool first,
