@@ -479,10 +479,12 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
 // Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-122 used, no
 // scratch): with the slot pipeline the compiler would otherwise take more and
 // run three, which measured no faster than the serial slots. Node-rebuilt
-// geometry: three waves (no scratch); held to four it spills to scratch and
-// runs 0.59 ms instead of 0.38 ms on the 4.7M-cell channel.
+// geometry, measured on the 4.7M-cell channel: Venkatakrishnan at four waves
+// (128 VGPRs, 20 bytes per lane of scratch: 0.364 ms against 0.368 ms at three
+// waves, 146 VGPRs), Barth-Jespersen at three (140 VGPRs, no scratch; at four
+// it keeps 28 bytes in scratch and runs 0.366 ms against 0.353 ms).
 template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE && !VENKAT ? 3 : 4)))
 k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
     const int tile = tile_of<WG>(m, ntiles);
     if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
