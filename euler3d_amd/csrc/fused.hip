@@ -223,6 +223,59 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long mask, int j) {
     return pos;
 }
 
+// The ghost slots of a wavefront dealt to its active lanes, for code that
+// only boundary cells run: slot k of a lane is a task where bit k of the
+// lane's slot set is set. Tasks are numbered slot by slot, lanes in order
+// within a slot, and handed to the active lanes in rank order, nact at a
+// time, so one pass of the code serves up to nact ghosts.
+template <int MAXS>
+struct SlotTasks {
+    unsigned long long mask[MAXS];
+    int first[MAXS + 1];
+    int rank, nact;
+    __device__ __forceinline__ explicit SlotTasks(unsigned slots) {
+        const int lane = __lane_id();
+        const unsigned long long act = __ballot(1);
+        rank = __popcll(act & ((1ull << lane) - 1));
+        nact = __popcll(act);
+        first[0] = 0;
+#pragma unroll
+        for (int k = 0; k < MAXS; ++k) {
+            mask[k] = __ballot((slots >> k) & 1u);
+            first[k + 1] = first[k] + __popcll(mask[k]);
+        }
+    }
+    __device__ __forceinline__ int count() const { return first[MAXS]; }  // the same on every lane
+    // slot and owning lane of task t < count()
+    // (selects with constant indices only: an array indexed at run time
+    // would be placed in scratch memory)
+    __device__ __forceinline__ void at(int t, int& k, int& src) const {
+        k = 0;
+        int base = 0;
+        unsigned long long msk = mask[0];
+#pragma unroll
+        for (int kk = 1; kk < MAXS; ++kk)
+            if (t >= first[kk]) {
+                k = kk;
+                base = first[kk];
+                msk = mask[kk];
+            }
+        src = nth_set_bit(msk, t - base);
+    }
+};
+
+// Lane src's value of v[k] for this lane's k (every lane takes part).
+template <int MAXS>
+__device__ __forceinline__ int shfl_slot(const int (&v)[MAXS], int k, int src) {
+    int r = __shfl(v[0], src, 64);
+#pragma unroll
+    for (int kk = 1; kk < MAXS; ++kk) {
+        const int x = __shfl(v[kk], src, 64);
+        if (k == kk) r = x;
+    }
+    return r;
+}
+
 // One cell of k_interp. FULL: every cell of the wavefront has MAXS faces (a
 // tet mesh with MAXS = 4 always does), so the per-slot tests drop out and the
 // face slots are straight-line code whose loads the compiler can overlap;
@@ -425,18 +478,8 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
     // the wavefront instead of one divergent pass per face slot. Same
     // operands, same records.
     {
-        const int lane = __lane_id();
-        const unsigned long long act = __ballot(1);
-        const int rank = __popcll(act & ((1ull << lane) - 1)), nact = __popcll(act);
-        unsigned long long slot_mask[MAXS];
-        int first[MAXS + 1];
-        first[0] = 0;
-#pragma unroll
-        for (int k = 0; k < MAXS; ++k) {
-            slot_mask[k] = __ballot((ghost_slots >> k) & 1u);
-            first[k + 1] = first[k] + __popcll(slot_mask[k]);
-        }
-        const int ntask = first[MAXS];  // the same on every lane
+        const SlotTasks<MAXS> tasks(ghost_slots);
+        const int ntask = tasks.count(), lane = __lane_id();
         if (ntask > 0) {
 #pragma unroll
             for (int i = 0; i < 15; ++i) r_lds[i][t] = g[i];
@@ -447,30 +490,18 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
 #pragma unroll 1
-        for (int t0 = 0; t0 < ntask; t0 += nact) {
-            const int task = t0 + rank;
-            int k = 0;
-#pragma unroll
-            for (int kk = 1; kk < MAXS; ++kk)
-                if (task >= first[kk]) k = kk;
-            unsigned long long msk = slot_mask[0];
-#pragma unroll
-            for (int kk = 1; kk < MAXS; ++kk)
-                if (k == kk) msk = slot_mask[kk];
+        for (int t0 = 0; t0 < ntask; t0 += tasks.nact) {
+            const int task = t0 + tasks.rank;
             const bool mine = task < ntask;
-            const int src = mine ? nth_set_bit(msk, task - first[k]) : lane;
+            int k = 0, src = lane;
+            if (mine) tasks.at(task, k, src);
             double gs[15], ws[5];
 #pragma unroll
             for (int i = 0; i < 15; ++i) gs[i] = r_lds[i][t - lane + src];
 #pragma unroll
             for (int v = 0; v < 5; ++v) ws[v] = __shfl(wc[v], src, 64);
             const double es = __shfl(eps2, src, 64);
-            int nb = __shfl(nbv[0], src, 64);
-#pragma unroll
-            for (int kk = 1; kk < MAXS; ++kk) {
-                const int x = __shfl(nbv[kk], src, 64);
-                if (k == kk) nb = x;
-            }
+            const int nb = shfl_slot<MAXS>(nbv, k, src);
             if (mine) ghost_face_state<VENKAT>(m, s, k, c - lane + src, nb, gs, ws, es, omega);
         }
     }
@@ -612,26 +643,48 @@ __global__ void __launch_bounds__(WG) k_flux(DMesh m, DState s, Gas gas, double 
             }
         }
         // ghosts of this cell: BC from the new interior state, or carried over
-#pragma unroll
-        for (int k = 0; k < MAXS; ++k)
-            if (k < nf) {
-                const size_t rec = (size_t)(mtv[k] & 0x7fffffff);
-                if (rec < ghost0) continue;
-                const int gi = (int)(rec - ghost0);
-                const int gcell = NRp + gi;
-                const int kind = m.g_kind[gi];
-                double gw[5];
-                if (kind & GK_BC_MASK) {
-                    double n[3];
-                    load3(m.g_n, m.NG, gi, n);  // S / area
-                    if ((kind & GK_BC_MASK) == GK_SLIP) ghost_slip(wnew, n, gw);
-                    else ghost_farfield(gas.gamma, wnew, fs, n, gw);
-                } else {
-                    load_w(s.W, m.NA, gcell, gw);
-                }
-#pragma unroll
-                for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + gcell] = gw[v];
+        auto ghost_bc = [&](int gi, const double wi[5]) {
+            const int gcell = NRp + gi;
+            const int kind = m.g_kind[gi];
+            double gw[5];
+            if (kind & GK_BC_MASK) {
+                double n[3];
+                load3(m.g_n, m.NG, gi, n);  // S / area
+                if ((kind & GK_BC_MASK) == GK_SLIP) ghost_slip(wi, n, gw);
+                else ghost_farfield(gas.gamma, wi, fs, n, gw);
+            } else {
+                load_w(s.W, m.NA, gcell, gw);
             }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) s.Wn[(size_t)v * m.NA + gcell] = gw[v];
+        };
+        if (FIRST) {  // slot by slot (measured 2 % faster than one pass for this variant)
+#pragma unroll
+            for (int k = 0; k < MAXS; ++k)
+                if (k < nf) {
+                    const size_t rec = (size_t)(mtv[k] & 0x7fffffff);
+                    if (rec >= ghost0) ghost_bc((int)(rec - ghost0), wnew);
+                }
+        } else {  // one pass, the ghost slots dealt to the active lanes (1-1.5 % faster)
+            unsigned ghost_slots = 0;
+#pragma unroll
+            for (int k = 0; k < MAXS; ++k)
+                if (k < nf && (size_t)(mtv[k] & 0x7fffffff) >= ghost0) ghost_slots |= 1u << k;
+            const SlotTasks<MAXS> tasks(ghost_slots);
+            const int ntask = tasks.count(), lane = __lane_id();
+#pragma unroll 1
+            for (int t0 = 0; t0 < ntask; t0 += tasks.nact) {
+                const int task = t0 + tasks.rank;
+                const bool mine = task < ntask;
+                int k = 0, src = lane;
+                if (mine) tasks.at(task, k, src);
+                double wi[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) wi[v] = __shfl(wnew[v], src, 64);  // the interior cell's new state
+                const int mt = shfl_slot<MAXS>(mtv, k, src);
+                if (mine) ghost_bc((int)((size_t)(mt & 0x7fffffff) - ghost0), wi);
+            }
+        }
     }
     if (LAST) {  // per-wavefront partial sums of the squared residual (fixed tree: deterministic)
 #pragma unroll
