@@ -507,7 +507,7 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
     }
 }
 
-// Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-122 used, no
+// Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-128 used, no
 // scratch): with the slot pipeline the compiler would otherwise take more and
 // run three, which measured no faster than the serial slots. Node-rebuilt
 // geometry, measured on the 4.7M-cell channel: Venkatakrishnan at four waves
