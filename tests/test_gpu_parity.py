@@ -189,6 +189,25 @@ def test_fused_medium_mesh_matches_oracle(scheme):
     ctx.close()
 
 
+@pytest.mark.parametrize("scheme", ["ausm2_lsq_bj", "roe2_gg_bj", "roe2_lsq_vk"])
+def test_hex_mesh_schemes_match_oracle(scheme):
+    """The hexahedral NACA0012 fixture mesh (six face slots per cell, up to 192
+    ghost slots per wavefront) with the gradient/limiter pairs its golden run
+    does not use, against the oracle."""
+    meta = dict(mach=0.8, aoa=1.25, flux=0, order=2, grad=0, lim=1, omega=1e-16, k=5.0, stages=5, iters=4)
+    if scheme == "ausm2_lsq_bj":
+        meta.update(flux=1, grad=1, lim=0)
+    elif scheme == "roe2_gg_bj":
+        meta.update(lim=0, omega=0.01)
+    elif scheme == "roe2_lsq_vk":
+        meta.update(grad=1)
+    mesh = mesh_for(Golden("naca_hex_roe2_gg_vk"))
+    ctx, hist, S, rh = run_both(mesh, meta, meta["iters"])
+    assert_states_equal(ctx, S)
+    assert_monitors_close(hist, rh)
+    ctx.close()
+
+
 @pytest.mark.parametrize("tiles", ["linear", "xcd_contiguous", "xcd_chunks"])
 def test_fused_tile_orders_match_oracle(tiles):
     """Every workgroup-to-tile order gives the oracle's values: 48x16x16 hexes
