@@ -171,22 +171,21 @@ __device__ __forceinline__ void ausm_diss(const Gas& g, const double wl[5], cons
 }
 
 // One face limiter value (Venkatakrishnan.cpp:227-238 / BarthJespersen.cpp:225-245).
-// Barth-Jespersen's d2 > 0 and d2 < 0 branches are one division over dmax or
-// dmin: the bound is selected first and divided once, so a wavefront whose
-// lanes differ in the sign of d2 does not run both (4.7M-cell channel: interp
-// 0.369 -> 0.357 ms); every lane gets the value of its own branch (1 where d2
-// is 0 or NaN, as there). Venkatakrishnan keeps the two branches: selected the
-// same way it measured slower (0.372 -> 0.385 ms).
+// The reference's d2 > 0 and d2 < 0 branches differ only in the bound (dmax or
+// dmin): the bound is selected first and the one expression evaluated, so a
+// wavefront whose lanes differ in the sign of d2 does not run both, and every
+// lane gets the value of its own branch (1 where d2 is 0 or NaN, as there).
+// On the 4.7M-cell channel: Barth-Jespersen interp 0.369 -> 0.357 ms;
+// Venkatakrishnan the same over the first 100 iterations (uniform signs) and
+// 0.430 -> 0.395 ms by iteration 1,000, where the flow has developed and the
+// signs mix within wavefronts.
 template <bool VENKAT>
 __device__ __forceinline__ double face_limiter(double d2, double dmax, double dmin, double eps2, double omega) {
     if (VENKAT) {
-        if (d2 > 0.0)
-            return (1.0 / d2) * ((dmax * dmax + eps2) * d2 + 2.0 * d2 * d2 * dmax) /
-                   (dmax * dmax + 2.0 * d2 * d2 + dmax * d2 + eps2);
-        if (d2 < 0.0)
-            return (1.0 / d2) * ((dmin * dmin + eps2) * d2 + 2.0 * d2 * d2 * dmin) /
-                   (dmin * dmin + 2.0 * d2 * d2 + dmin * d2 + eps2);
-        return 1.0;
+        const double dm = d2 > 0.0 ? dmax : dmin;
+        const double phi = (1.0 / d2) * ((dm * dm + eps2) * d2 + 2.0 * d2 * d2 * dm) /
+                           (dm * dm + 2.0 * d2 * d2 + dm * d2 + eps2);
+        return (d2 > 0.0 || d2 < 0.0) ? phi : 1.0;
     } else {
         d2 = (copysign(1.0, d2)) * (fabs(d2) + omega);
         const double dm = d2 > 0.0 ? dmax : dmin;
