@@ -7,7 +7,11 @@ second-order Roe with Green-Gauss gradients and the Venkatakrishnan limiter
 (K=5), M=0.675, 5-stage RK, local CFL 1.5, FP64. A step is one RK iteration
 (5 stages, the interface exchange and the residual/force monitors) over the
 whole mesh, launched through e3d_iterate; the mesh and state are resident in
-HBM before the timed region starts.
+HBM before the timed region starts. By default the timed region is the whole
+1,000-iteration run from the freestream start (after 3 untimed iterations):
+the stage kernels grow slower as the flow develops (the limiter's and the
+entropy fix's branches mix within wavefronts), so a short run at the start
+would overstate the rate.
 
 metric: Mcell-RK-stage/s = real cells x RK stages x steps / wall time.
 roofline: the kernel with the largest share of the timed region (interp:
@@ -354,7 +358,7 @@ def warm_start_leg():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=192)
     ap.add_argument("--ny", type=int, default=64)
