@@ -9,9 +9,9 @@ second-order Roe with Green-Gauss gradients and the Venkatakrishnan limiter
 whole mesh, launched through e3d_iterate; the mesh and state are resident in
 HBM before the timed region starts. By default the timed region is the whole
 1,000-iteration run from the freestream start (after 3 untimed iterations):
-the stage kernels grow slower as the flow develops (the limiter's and the
-entropy fix's branches mix within wavefronts), so a short run at the start
-would overstate the rate.
+the flux kernels grow slower as the flow develops (the lanes of a wavefront
+stop agreeing on the branches of the Roe entropy fix), so a short run at the
+start would overstate the rate.
 
 metric: Mcell-RK-stage/s = real cells x RK stages x steps / wall time.
 roofline: the kernel with the largest share of the timed region (interp:

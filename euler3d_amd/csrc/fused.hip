@@ -510,12 +510,13 @@ __device__ __forceinline__ void interp_cell(const DMesh& m, const DState& s, dou
 // Streamed geometry: four waves per SIMD (at most 128 VGPRs; 120-128 used, no
 // scratch): with the slot pipeline the compiler would otherwise take more and
 // run three, which measured no faster than the serial slots. Node-rebuilt
-// geometry, measured on the 4.7M-cell channel: Venkatakrishnan at four waves
-// (128 VGPRs, 20 bytes per lane of scratch: 0.364 ms against 0.368 ms at three
-// waves, 146 VGPRs), Barth-Jespersen at three (140 VGPRs, no scratch; at four
-// it keeps 28 bytes in scratch and runs 0.366 ms against 0.353 ms).
+// geometry: three waves (140-154 VGPRs, no scratch). Measured on the 4.7M-cell
+// channel: at four the Venkatakrishnan form keeps 20 bytes per lane in
+// scratch and runs 0.392 against 0.388 ms over the first 100 iterations
+// (0.364 against 0.368 ms while its limiter still branched on the sign),
+// Barth-Jespersen keeps 28 bytes and runs 0.366 against 0.353 ms.
 template <int GRAD, bool VENKAT, int MAXS, int WG, bool NODE>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE && !VENKAT ? 3 : 4)))
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NODE ? 3 : 4)))
 k_interp(DMesh m, DState s, double k3, double omega, int ntiles) {
     const int tile = tile_of<WG>(m, ntiles);
     if (tile >= ntiles || !instance_active(m, cell_instance(m, tile * WG))) return;
